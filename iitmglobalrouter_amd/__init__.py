@@ -1,0 +1,251 @@
+"""MI355X (gfx950) routing hot path of the IITM 3-D global router.
+
+The per-batch path (L-pattern costing -> root-pair Boruvka tree -> Fenwick
+demand commit) runs as hand-written HIP kernels inside ``libiitmgr.so``; this
+module is the host-side mirror of the reference's router loop
+(``TEST_LPAT::test_gpu_lpat``, l_pattern_files/test_lpat.hpp:231-476) over the
+C ABI in ``include/iitmgr.h``.  There is no CPU fallback: without the built
+library or a gfx950 device every routing call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "load_library", "RouterError", "Router", "gen_grid", "gen_nets", "partition_nets",
+    "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
+    "WORKLOADS", "device_count",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libiitmgr.so")
+
+COST_REF_BIT, COST_REF_INC, COST_LIT_EXP = 0, 1, 2
+COMMIT_REF_QUIRK, COMMIT_FIXED = 0, 1
+K_LPAT, K_TREE, K_REFRESH = 0, 1, 2
+
+# Workload families of the synthetic generator: grid shape and net count.
+WORKLOADS = {
+    1: dict(X=128, Y=128, Z=6, nets=5000),        # plain method, CPU-runnable
+    2: dict(X=800, Y=800, Z=10, nets=150000),     # mid-chip, 5% blocked cells
+    3: dict(X=2000, Y=2000, Z=10, nets=1500000),  # heavy-tailed pin counts
+    4: dict(X=800, Y=800, Z=10, nets=150000),     # same generator as 2
+    5: dict(X=2000, Y=2000, Z=10, nets=1500000),  # 2000^2 with cut capacity + hotspots
+}
+
+
+class RouterError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("X", ctypes.c_int32), ("Y", ctypes.c_int32), ("Z", ctypes.c_int32),
+                ("cost_model", ctypes.c_int32), ("commit_mode", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+
+_SIGS = {
+    "gr_version": (ctypes.c_char_p, []),
+    "gr_device_count": (ctypes.c_int, []),
+    "gr_lut_rmin": (ctypes.c_int, []),
+    "gr_lut_size": (ctypes.c_int, []),
+    "gr_last_error": (ctypes.c_char_p, [_P]),
+    "gr_create": (ctypes.c_int, [ctypes.POINTER(_Desc), _P, ctypes.POINTER(_P)]),
+    "gr_destroy": (None, [_P]),
+    "gr_set_nets": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "gr_route_batch": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
+    "gr_set_schedule": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "gr_route_schedule": (ctypes.c_int, [_P, _I32, _I32]),
+    "gr_sync": (ctypes.c_int, [_P]),
+    "gr_get_schedule_outputs": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gr_schedule_slots": (_I64, [_P]),
+    "gr_reset_demand": (ctypes.c_int, [_P]),
+    "gr_get_demand": (ctypes.c_int, [_P, _P]),
+    "gr_get_prefix": (ctypes.c_int, [_P, _P]),
+    "gr_last_pairs": (_I64, [_P]),
+    "gr_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
+    "gr_get_lut": (ctypes.c_int, [_P, _P]),
+    "gr_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gr_kernel_stats": (ctypes.c_int, [_P, _P, _P]),
+    "gr_gen_grid": (ctypes.c_int, [_I32, _U64, _I32, _I32, _I32, _P]),
+    "gr_gen_nets": (ctypes.c_int, [_I32, _U64, _I32, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "gr_partition_nets": (ctypes.c_int, [_I32, _I32, _I32, _I32, _P, _P, _I32, _P, _P]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libiitmgr.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check(rc, ctx=None):
+    if rc < 0:
+        lib = load_library()
+        msg = lib.gr_last_error(ctx).decode()
+        raise RouterError(f"error {rc}: {msg}")
+    return rc
+
+
+def device_count():
+    return load_library().gr_device_count()
+
+
+def gen_grid(kind, seed, X, Y, Z):
+    """Capacity cube (int32, WIRE_ENC order) of a synthetic workload family."""
+    lib = load_library()
+    cap = np.empty(X * Y * Z, dtype=np.int32)
+    _check(lib.gr_gen_grid(kind, seed, X, Y, Z, _ptr(cap)))
+    return cap
+
+
+def gen_nets(kind, seed, X, Y, Z, n_nets):
+    """Nets of a synthetic workload: (pin_pfx int64[n+1], pin_enc int32[...])."""
+    lib = load_library()
+    nn = ctypes.c_int32()
+    npins = ctypes.c_int64()
+    _check(lib.gr_gen_nets(kind, seed, X, Y, Z, n_nets, ctypes.byref(nn), ctypes.byref(npins), None, None))
+    pfx = np.empty(nn.value + 1, dtype=np.int64)
+    pins = np.empty(npins.value, dtype=np.int32)
+    _check(lib.gr_gen_nets(kind, seed, X, Y, Z, n_nets, ctypes.byref(nn), ctypes.byref(npins),
+                           _ptr(pfx), _ptr(pins)))
+    return pfx, pins
+
+
+def partition_nets(X, Y, Z, pin_pfx, pin_enc, num_xdim_parts=4):
+    """Batches of nets with pairwise disjoint bounding boxes (net_part.hpp:167-379)."""
+    lib = load_library()
+    n = len(pin_pfx) - 1
+    ofst = np.empty(n + 1, dtype=np.int32)
+    ids = np.empty(max(n, 1), dtype=np.int32)
+    nb = _check(lib.gr_partition_nets(X, Y, Z, n, _ptr(np.ascontiguousarray(pin_pfx, dtype=np.int64)),
+                                      _ptr(np.ascontiguousarray(pin_enc, dtype=np.int32)), num_xdim_parts,
+                                      _ptr(ofst), _ptr(ids)))
+    return ofst[: nb + 1].copy(), ids[:n].copy()
+
+
+class Router:
+    """One routing context on one gfx950 device.
+
+    Mirrors the router thread of the reference (test_lpat.hpp:231-476): the
+    grid and nets are loaded once, then batches are routed in order.
+    """
+
+    def __init__(self, X, Y, Z, cap, cost_model=COST_REF_BIT, commit_mode=COMMIT_REF_QUIRK, device=0):
+        self.lib = load_library()
+        self.X, self.Y, self.Z = X, Y, Z
+        cap = np.ascontiguousarray(cap, dtype=np.int32)
+        if cap.size != X * Y * Z:
+            raise ValueError("capacity cube has the wrong size")
+        d = _Desc(X, Y, Z, cost_model, commit_mode, device)
+        h = ctypes.c_void_p()
+        rc = self.lib.gr_create(ctypes.byref(d), _ptr(cap), ctypes.byref(h))
+        if rc < 0:
+            raise RouterError(f"gr_create error {rc}: {self.lib.gr_last_error(None).decode()}")
+        self.h = h
+        self.deg = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, rc):
+        return _check(rc, self.h)
+
+    def set_nets(self, pin_pfx, pin_enc):
+        self.pin_pfx = np.ascontiguousarray(pin_pfx, dtype=np.int64)
+        self.pin_enc = np.ascontiguousarray(pin_enc, dtype=np.int32)
+        self.deg = np.diff(self.pin_pfx)
+        self._c(self.lib.gr_set_nets(self.h, len(self.pin_pfx) - 1, _ptr(self.pin_pfx), _ptr(self.pin_enc)))
+
+    def route_batch(self, net_ids):
+        """Route one batch; returns (edges int64[sum deg], lpat int32[sum deg], net_cost f32[n])."""
+        ids = np.ascontiguousarray(net_ids, dtype=np.int32)
+        slots = int(self.deg[ids].sum()) if len(ids) else 0
+        edges = np.empty(slots, dtype=np.int64)
+        lpat = np.empty(slots, dtype=np.int32)
+        cost = np.empty(len(ids), dtype=np.float32)
+        self._c(self.lib.gr_route_batch(self.h, _ptr(ids), len(ids), _ptr(edges), _ptr(lpat), _ptr(cost)))
+        return edges, lpat, cost
+
+    def set_schedule(self, batch_ofst, net_ids):
+        self.sched_ofst = np.ascontiguousarray(batch_ofst, dtype=np.int32)
+        self.sched_ids = np.ascontiguousarray(net_ids, dtype=np.int32)
+        self._c(self.lib.gr_set_schedule(self.h, len(self.sched_ofst) - 1, _ptr(self.sched_ofst),
+                                         _ptr(self.sched_ids)))
+
+    def route_schedule(self, first=0, count=None):
+        if count is None:
+            count = len(self.sched_ofst) - 1 - first
+        self._c(self.lib.gr_route_schedule(self.h, first, count))
+
+    def sync(self):
+        self._c(self.lib.gr_sync(self.h))
+
+    def schedule_outputs(self):
+        slots = self.lib.gr_schedule_slots(self.h)
+        edges = np.empty(slots, dtype=np.int64)
+        lpat = np.empty(slots, dtype=np.int32)
+        cost = np.empty(len(self.sched_ids), dtype=np.float32)
+        self._c(self.lib.gr_get_schedule_outputs(self.h, _ptr(edges), _ptr(lpat), _ptr(cost)))
+        return edges, lpat, cost
+
+    def reset_demand(self):
+        self._c(self.lib.gr_reset_demand(self.h))
+
+    def demand(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
+        self._c(self.lib.gr_get_demand(self.h, _ptr(out)))
+        return out
+
+    def prefix(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        self._c(self.lib.gr_get_prefix(self.h, _ptr(out)))
+        return out
+
+    def pair_costs(self):
+        n = self.lib.gr_last_pairs(self.h)
+        cost = np.empty(n, dtype=np.float32)
+        pat = np.empty(n, dtype=np.int16)
+        self._c(self.lib.gr_get_pair_costs(self.h, _ptr(cost), _ptr(pat)))
+        return cost, pat
+
+    def lut(self):
+        out = np.empty(self.lib.gr_lut_size(), dtype=np.float32)
+        self._c(self.lib.gr_get_lut(self.h, _ptr(out)))
+        return out
+
+    def enable_timing(self, on=True):
+        self._c(self.lib.gr_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        n = np.zeros(3, dtype=np.int64)
+        ms = np.zeros(3, dtype=np.float64)
+        self._c(self.lib.gr_kernel_stats(self.h, _ptr(n), _ptr(ms)))
+        return n, ms

@@ -1,0 +1,128 @@
+// Device-side building blocks of the routing hot path (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace grd {
+
+constexpr int kRMin = -256;
+constexpr int kLut = 512;  // r in [-256, 255]
+constexpr int kSmallDeg = 64;  // nets up to this many pins: one wavefront per net
+constexpr int kLargeDeg = 2048;  // LDS-resident tree for nets up to this many pins
+
+struct Geo {
+  int X, Y, Z, Z2;
+  uint32_t XY;
+};
+
+// Grid encoding, util.hpp:38-44 (cells fit in int32: CSIZE < 2^31).
+__device__ __forceinline__ uint32_t enc(const Geo& g, int z, int x, int y) {
+  return (z & 1) ? (uint32_t)z * g.XY + (uint32_t)x * g.Y + y : (uint32_t)z * g.XY + (uint32_t)y * g.X + x;
+}
+__device__ __forceinline__ void dec(const Geo& g, uint32_t v, int& z, int& x, int& y) {
+  z = (int)(v / g.XY);
+  uint32_t r = v - (uint32_t)z * g.XY;
+  if (z & 1) { x = (int)(r / g.Y); y = (int)(r - (uint32_t)x * g.Y); }
+  else       { y = (int)(r / g.X); x = (int)(r - (uint32_t)y * g.X); }
+}
+
+// Orderable key of an fp32 cost: -0 and +0 map to the same key, negatives
+// below positives, so (key << 32 | index) orders lexicographically by
+// (cost, index).
+__device__ __forceinline__ uint32_t ord_key(float f) {
+  f = f + 0.0f;
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_cost(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+
+// Materialised Fenwick prefix Q(z, line, pos) (bit_util.hpp:19-35); pos = -1 -> 0.
+__device__ __forceinline__ float Q(const float* __restrict__ pfx, const Geo& g, int z, int line, int pos) {
+  if (pos < 0) return 0.0f;
+  uint32_t L = (z & 1) ? g.Y : g.X;
+  return pfx[(uint32_t)z * g.XY + (uint32_t)line * L + pos];
+}
+
+// Via stack cost, bit_util.hpp:108-117: each layer i in [min, max) adds the
+// single-cell range query Q(pos) - Q(pos - 1), in ascending layer order.
+__device__ __forceinline__ void via_cost(const float* __restrict__ pfx, const Geo& g, int za, int zb, int x,
+                                         int y, float& cost) {
+  int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
+  for (int i = lo; i < hi; ++i) {
+    if (i & 1) cost += Q(pfx, g, i, x, y) - Q(pfx, g, i, x, y - 1);
+    else       cost += Q(pfx, g, i, y, x) - Q(pfx, g, i, y, x - 1);
+  }
+}
+
+// Wire cost along layer z's direction, inclusive of both ends, bit_util.hpp:91-107.
+__device__ __forceinline__ void wire_cost(const float* __restrict__ pfx, const Geo& g, int z, int& xs, int& ys,
+                                          int x2, int y2, float& cost) {
+  if (z & 1) {
+    if (ys < y2)      cost += Q(pfx, g, z, xs, y2) - Q(pfx, g, z, xs, ys - 1);
+    else if (ys > y2) cost += Q(pfx, g, z, xs, ys) - Q(pfx, g, z, xs, y2 - 1);
+    ys = y2;
+  } else {
+    if (xs < x2)      cost += Q(pfx, g, z, ys, x2) - Q(pfx, g, z, ys, xs - 1);
+    else if (xs > x2) cost += Q(pfx, g, z, ys, xs) - Q(pfx, g, z, ys, x2 - 1);
+    xs = x2;
+  }
+}
+
+// Cost of pattern p for the pair src -> dst (bit_util.hpp:131-179).  Returns
+// false for an invalid pattern (a = 0, b' >= Z, or not ending at dst).
+__device__ __forceinline__ bool pattern_cost(const float* __restrict__ pfx, const Geo& g, int z1, int x1, int y1,
+                                             int z2, int x2, int y2, int p, float& out) {
+  int a = p / g.Z2, b = p - a * g.Z2;
+  if (a == 0) return false;
+  bool D = (x1 != x2) && (y1 != y2);
+  int zs = z1, xs = x1, ys = y1;
+  float cost = 0.0f;
+  if (zs != a) { via_cost(pfx, g, zs, a, xs, ys, cost); zs = a; }
+  wire_cost(pfx, g, zs, xs, ys, x2, y2, cost);
+  int bp = (zs & 1) ? 2 * b : 2 * b + 1;
+  if (D && bp != 0) {
+    if (bp >= g.Z) return false;
+    via_cost(pfx, g, zs, bp, xs, ys, cost);
+    zs = bp;
+    wire_cost(pfx, g, zs, xs, ys, x2, y2, cost);
+  }
+  if (zs != z2) { via_cost(pfx, g, zs, z2, xs, ys, cost); zs = z2; }
+  if (xs != x2 || ys != y2) return false;
+  out = cost;
+  return true;
+}
+
+// Pair index o = j(j-1)/2 + k, k < j (test_lpat.hpp:316-331) -> (j, k).
+__device__ __forceinline__ void pair_decode(uint32_t o, int& j, int& k) {
+  int jj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)o)) * 0.5);
+  while ((uint64_t)jj * (jj - 1) / 2 > o) --jj;
+  while ((uint64_t)(jj + 1) * jj / 2 <= o) ++jj;
+  j = jj;
+  k = (int)(o - (uint32_t)((uint64_t)jj * (jj - 1) / 2));
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, m, 64);
+  hi = __shfl_xor(hi, m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint64_t o = shfl_xor_u64(v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace grd

@@ -1,0 +1,199 @@
+/*
+ * iitmgr.h - C ABI of the MI355X (gfx950) routing hot path.
+ *
+ * The reference router has no API for this path: its per-batch body lives in
+ * TEST_LPAT::test_gpu_lpat (l_pattern_files/test_lpat.hpp:280-450) and talks to
+ * its callers through namespace globals (common.h:313-349, mst_common.h:76-88).
+ * Each entry point below replaces one piece of that body or of its one-time
+ * set-up, as cited per function.  Only plain C types cross this boundary.
+ *
+ * Conventions
+ *   - Every int-returning call returns 0 (or a non-negative count) on success
+ *     and a negative GR_E* code on failure; gr_last_error() explains it.
+ *     Nothing aborts the process (the reference asserts, common.h:140-146).
+ *   - Grid cells use the reference's WIRE_ENC order (util.hpp:38-44):
+ *       enc(z,x,y) = z*X*Y + (z odd ? x*Y + y : y*X + x)
+ *     Layer 0 is horizontal; layers alternate H/V.
+ *   - Pins of a net are deduplicated and ascending by enc
+ *     (read_data.hpp:413-432); nets need at least 2 pins.
+ *   - A batch is a list of nets whose closed 2-D pin bounding boxes are
+ *     pairwise disjoint (part_net/net_part.hpp:167-235).  Batches must be
+ *     routed in order: batch b+1 reads the demand committed by batch b.
+ *   - Output slots: net i of a batch owns deg(i) consecutive slots; the first
+ *     deg-1 hold tree edges encoded (i << 30) | (on1 << 15) | on2 and the last
+ *     holds -1 (gmst_util.hpp:322-339).  on1 is the root of the component that
+ *     chose the edge, on2 the root of the component it reached; both are pin
+ *     indices of the net.  The routed pair is (max(on1,on2), min(on1,on2)).
+ *   - Pattern code p = a*Z2 + b, Z2 = ceil(Z/2) (lpattern_util.hpp:43-53):
+ *     first wire on layer a, second on b' = (a odd ? 2b : 2b+1).
+ *   - One context is driven by one host thread.  The library owns all device
+ *     memory and reuses it across batches.
+ */
+#ifndef IITMGR_H
+#define IITMGR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define GR_OK 0
+#define GR_EINVAL -1    /* bad argument */
+#define GR_ENODEV -2    /* no usable gfx950 device / HIP runtime error */
+#define GR_ENOMEM -3    /* device allocation failed */
+#define GR_ESTATE -4    /* call out of order (e.g. routing before gr_set_nets) */
+#define GR_ERANGE -5    /* size beyond a documented limit */
+
+/* cost models: all map integer remaining capacity r = cap - demand to fp32 */
+#define GR_COST_REF_BIT 0   /* exp(-r/2): bit_util.hpp:13-15 (paper -bit_pmodel run) */
+#define GR_COST_REF_INC 1   /* exp(-(r-1)/2) - exp(-r/2): util.hpp:127-129 */
+#define GR_COST_LIT_EXP 2   /* exp(d + 1 - c) = exp(1 - r): README.md:10 */
+
+/* commit modes */
+#define GR_COMMIT_REF_QUIRK 0 /* reproduce the unguarded first wire, mst_bit_commit.hpp:216 */
+#define GR_COMMIT_FIXED 1     /* mark only the cells of the route */
+
+typedef struct gr_ctx gr_ctx;
+
+typedef struct {
+  int32_t X, Y, Z;      /* grid; Z >= 3; X, Y <= 8192 */
+  int32_t cost_model;   /* GR_COST_* */
+  int32_t commit_mode;  /* GR_COMMIT_* */
+  int32_t device;       /* HIP device ordinal */
+} gr_grid_desc;
+
+/* Library version string. */
+const char* gr_version(void);
+
+/* Number of gfx950 devices visible (0 without a GPU); never fails. */
+int gr_device_count(void);
+
+/*
+ * Create a context and upload the capacity cube (CSIZE = X*Y*Z int32 values,
+ * WIRE_ENC order; demand starts at 0).  Builds the per-run Fenwick cost state.
+ * Replaces: read_cap_file's cube upload (read_data.hpp:554-562) and the
+ * cost-tree build BIT_UTIL_SPACE::init_bit_data (bit_util.hpp:192-200),
+ * called from test_lpat.hpp:239-242.
+ */
+int gr_create(const gr_grid_desc* desc, const int32_t* cap, gr_ctx** out);
+
+/* Free every device and host resource of the context. */
+void gr_destroy(gr_ctx* ctx);
+
+/* Message for the last failing call on this context (or a global one if ctx is NULL). */
+const char* gr_last_error(const gr_ctx* ctx);
+
+/*
+ * Upload all nets: pin_pfx[n_nets+1] (int64 exclusive prefix of pin counts)
+ * and pin_enc[pin_pfx[n_nets]] (per net ascending, deduplicated, >= 2 pins,
+ * <= 32767 pins).  Replaces READ_DATA::add_net_gpu_data (read_data.hpp:486-524).
+ */
+int gr_set_nets(gr_ctx* ctx, int32_t n_nets, const int64_t* pin_pfx, const int32_t* pin_enc);
+
+/*
+ * Route one batch synchronously: L-pattern costing of every pin pair, root-pair
+ * Boruvka tree per net, commit of the tree's routes into demand, refresh of
+ * the cost state.  out_edges / out_lpat have sum(deg) slots, out_net_cost n
+ * entries (sum of the chosen minimum edge costs, reference net_dist).  Any of
+ * the outputs may be NULL.  Returns 0.
+ * Replaces one iteration of the batch loop, test_lpat.hpp:280-450: pair build
+ * (:293-346), BIT_UTIL_SPACE::get_gpu_lpat_dist (bit_util.hpp:118-191),
+ * MST_UTIL::get_block_net_map (mst_util.hpp:152-320) and
+ * MST_BIT_COMMIT::bit_commit_edges (mst_bit_commit.hpp:236-255), plus the
+ * D2H hand-off to the writer (mst_util.hpp:316-319).
+ */
+int gr_route_batch(gr_ctx* ctx, const int32_t* net_ids, int32_t n,
+                   int64_t* out_edges, int32_t* out_lpat, float* out_net_cost);
+
+/*
+ * Keep a whole batch schedule resident on the device: batch b covers
+ * net_ids[batch_ofst[b] .. batch_ofst[b+1]).  Replaces the batcher's per-batch
+ * uploads and scans (part_net/net_part.hpp:345-369).
+ */
+int gr_set_schedule(gr_ctx* ctx, int32_t n_batches, const int32_t* batch_ofst,
+                    const int32_t* net_ids);
+
+/*
+ * Enqueue batches [first, first+count) of the resident schedule on the
+ * context's stream without any host synchronisation.  Outputs stay on the
+ * device until gr_get_schedule_outputs.  Same per-batch semantics as
+ * gr_route_batch.  Replaces the router thread's while loop, test_lpat.hpp:280.
+ */
+int gr_route_schedule(gr_ctx* ctx, int32_t first, int32_t count);
+
+/* Wait for all enqueued work; returns a HIP error as GR_ENODEV. */
+int gr_sync(gr_ctx* ctx);
+
+/*
+ * Copy schedule outputs to the host: edges/lpat have sum over the schedule of
+ * deg slots (schedule order), net_cost one entry per scheduled net.
+ * Replaces the writer hand-off (mst_util.hpp:316-319).
+ */
+int gr_get_schedule_outputs(gr_ctx* ctx, int64_t* out_edges, int32_t* out_lpat, float* out_net_cost);
+
+/* Total slots (sum of deg) and nets of the resident schedule. */
+int64_t gr_schedule_slots(const gr_ctx* ctx);
+
+/* Restore demand to zero and rebuild the cost state (start a new routing pass). */
+int gr_reset_demand(gr_ctx* ctx);
+
+/* Integer demand per cell (CSIZE int32), the parity state. */
+int gr_get_demand(gr_ctx* ctx, int32_t* out);
+
+/* Materialised Fenwick prefix Q per cell (CSIZE fp32), for parity tests. */
+int gr_get_prefix(gr_ctx* ctx, float* out);
+
+/* Number of pin pairs in the last routed batch. */
+int64_t gr_last_pairs(const gr_ctx* ctx);
+
+/* Per-pair minimum cost and pattern of the last routed batch, batch pair order. */
+int gr_get_pair_costs(gr_ctx* ctx, float* cost, int16_t* pat);
+
+/* Cost LUT of the context: fills out[gr_lut_size()] for r = gr_lut_rmin() + i. */
+int gr_get_lut(const gr_ctx* ctx, float* out);
+int gr_lut_rmin(void);
+int gr_lut_size(void);
+
+/*
+ * Kernel timing: when enabled, every launch of the listed kernels is bracketed
+ * by HIP events on the launch stream.  gr_kernel_stats returns, per kernel
+ * id, the launch count and total milliseconds since the last enable.
+ * Kernel ids: 0 L-pattern costing, 1 tree+commit, 2 cost refresh.
+ */
+#define GR_K_LPAT 0
+#define GR_K_TREE 1
+#define GR_K_REFRESH 2
+#define GR_K_COUNT 3
+int gr_enable_timing(gr_ctx* ctx, int on);
+int gr_kernel_stats(gr_ctx* ctx, int64_t* launches, double* total_ms);
+
+/* ---- host side of the path (no device work) ---- */
+
+/*
+ * Synthetic workload generator (seeded, deterministic; spec in DESIGN.md).
+ * kind: 1..5 selects the workload family.  First call with NULL buffers to
+ * learn sizes (n_nets, n_pins); then call again with buffers.
+ * cap gets X*Y*Z values; pin_pfx n_nets+1; pin_enc n_pins.
+ */
+int gr_gen_grid(int32_t kind, uint64_t seed, int32_t X, int32_t Y, int32_t Z, int32_t* cap);
+int gr_gen_nets(int32_t kind, uint64_t seed, int32_t X, int32_t Y, int32_t Z, int32_t n_nets_req,
+                int32_t* n_nets, int64_t* n_pins, int64_t* pin_pfx, int32_t* pin_enc);
+
+/*
+ * Batch generation: greedy packing of nets with pairwise disjoint closed
+ * bounding boxes into batches, num_xdim_parts x-interval buckets per batch.
+ * Replaces NET_PART::part_nets / part_xdim (part_net/net_part.hpp:167-379).
+ * Outputs batch_ofst[n_batches+1] and net_ids[n_nets] (each batch sorted by
+ * (pin count, net id)).  Returns the number of batches.
+ */
+int gr_partition_nets(int32_t X, int32_t Y, int32_t Z, int32_t n_nets, const int64_t* pin_pfx,
+                      const int32_t* pin_enc, int32_t num_xdim_parts,
+                      int32_t* batch_ofst, int32_t* net_ids);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IITMGR_H */

@@ -1,0 +1,155 @@
+"""CPU oracle of the per-batch routing hot path (TEST INFRASTRUCTURE).
+
+A ctypes wrapper over liboracle.so (oracle.cpp, a C++ restatement of the
+reference's algorithm).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this package; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+_SIGS = {
+    "orc_lut": (None, [ctypes.c_int, _P]),
+    "orc_lut_rmin": (ctypes.c_int, []),
+    "orc_lut_size": (ctypes.c_int, []),
+    "orc_create": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    "orc_destroy": (None, [_P]),
+    "orc_set_nets": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+    "orc_get_demand": (ctypes.c_int, [_P, _P]),
+    "orc_get_pfx": (ctypes.c_int, [_P, _P]),
+    "orc_set_rem": (ctypes.c_int, [_P, _P]),
+    "orc_pattern_cost": (ctypes.c_float, [_P, _I32, _I32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "orc_pattern_cost_walk": (ctypes.c_double, [_P, _I32, _I32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "orc_best_pattern": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
+    "orc_edge_cells": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int, _P, ctypes.c_int]),
+    "orc_boruvka": (ctypes.c_int, [ctypes.c_int, _P, _P, _P]),
+    "orc_route_batch": (_I64, [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]),
+    "orc_last_pairs": (_I64, [_P]),
+    "orc_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def lut(model):
+    out = np.empty(lib().orc_lut_size(), dtype=np.float32)
+    lib().orc_lut(model, _ptr(out))
+    return out
+
+
+def boruvka(n, costs):
+    """Root-pair Boruvka on explicit pair costs: (edges[(on1, on2)], net_cost, rounds)."""
+    c = np.ascontiguousarray(costs, dtype=np.float32)
+    out = np.empty(n, dtype=np.int64)
+    nc = ctypes.c_float()
+    rounds = lib().orc_boruvka(n, _ptr(c), _ptr(out), ctypes.byref(nc))
+    edges = [(int(v >> 15), int(v & 0x7FFF)) for v in out[: n - 1]]
+    assert out[n - 1] == -1
+    return edges, nc.value, rounds
+
+
+class Oracle:
+    def __init__(self, X, Y, Z, cap, cost_model=0, commit_mode=0):
+        self.X, self.Y, self.Z = X, Y, Z
+        cap = np.ascontiguousarray(cap, dtype=np.int32)
+        self.h = lib().orc_create(X, Y, Z, cost_model, commit_mode, _ptr(cap))
+        if not self.h:
+            raise ValueError("bad oracle grid")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def enc(self, z, x, y):
+        X, Y = self.X, self.Y
+        return z * X * Y + (x * Y + y if z & 1 else y * X + x)
+
+    def set_nets(self, pin_pfx, pin_enc):
+        self.pin_pfx = np.ascontiguousarray(pin_pfx, dtype=np.int64)
+        self.pin_enc = np.ascontiguousarray(pin_enc, dtype=np.int32)
+        self.deg = np.diff(self.pin_pfx)
+        lib().orc_set_nets(self.h, len(self.pin_pfx) - 1, _ptr(self.pin_pfx), _ptr(self.pin_enc))
+
+    def set_rem(self, rem):
+        r = np.ascontiguousarray(rem, dtype=np.int32)
+        lib().orc_set_rem(self.h, _ptr(r))
+
+    def route_batch(self, net_ids, nthreads=0):
+        ids = np.ascontiguousarray(net_ids, dtype=np.int32)
+        slots = int(self.deg[ids].sum()) if len(ids) else 0
+        edges = np.empty(slots, dtype=np.int64)
+        lpat = np.empty(slots, dtype=np.int32)
+        cost = np.empty(len(ids), dtype=np.float32)
+        lib().orc_route_batch(self.h, _ptr(ids), len(ids), _ptr(edges), _ptr(lpat), _ptr(cost), nthreads)
+        return edges, lpat, cost
+
+    def demand(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
+        lib().orc_get_demand(self.h, _ptr(out))
+        return out
+
+    def prefix(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        lib().orc_get_pfx(self.h, _ptr(out))
+        return out
+
+    def pair_costs(self):
+        n = lib().orc_last_pairs(self.h)
+        cost = np.empty(n, dtype=np.float32)
+        pat = np.empty(n, dtype=np.int16)
+        lib().orc_get_pair_costs(self.h, _ptr(cost), _ptr(pat))
+        return cost, pat
+
+    def pattern_cost(self, src, dst, p):
+        v = ctypes.c_int()
+        c = lib().orc_pattern_cost(self.h, int(src), int(dst), int(p), ctypes.byref(v))
+        return (c if v.value else None)
+
+    def pattern_cost_walk(self, src, dst, p):
+        v = ctypes.c_int()
+        c = lib().orc_pattern_cost_walk(self.h, int(src), int(dst), int(p), ctypes.byref(v))
+        return (c if v.value else None)
+
+    def best_pattern(self, src, dst):
+        c = ctypes.c_float()
+        p = lib().orc_best_pattern(self.h, int(src), int(dst), ctypes.byref(c))
+        return p, c.value
+
+    def edge_cells(self, src, dst, p):
+        buf = np.empty(4096, dtype=np.int64)
+        n = lib().orc_edge_cells(self.h, int(src), int(dst), int(p), _ptr(buf), len(buf))
+        assert n <= len(buf)
+        return buf[:n].copy()

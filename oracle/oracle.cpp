@@ -1,0 +1,505 @@
+// CPU oracle for the per-batch routing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain C++ restatement of the
+// reference router's per-batch algorithm, used by tests/ as the checker, by
+// __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline leg
+// as the timed CPU baseline.  The product library (iitmglobalrouter_amd/) never
+// links, loads or calls it.
+//
+// What it restates (reference files are cited file:line):
+//   * grid encoding                      util.hpp:37-68
+//   * cost of a cell f(r)                bit_util.hpp:13-18, util.hpp:119-132
+//   * Fenwick run trees + range query    bit_util.hpp:19-55, 91-117
+//   * L-pattern enumeration/selection    bit_util.hpp:118-191 (CPU twin order:
+//                                        lpattern.hpp:98-160, lowest p wins)
+//   * pattern code p = a*Z2 + b          lpattern_util.hpp:43-53
+//   * pair order o = j(j-1)/2 + k        test_lpat.hpp:311-331, gmst_util.hpp:192-194
+//   * root-pair Boruvka                  gmst_util.hpp:164-341, mst_util.hpp:289-314
+//   * BIT commit cell marking            mst_bit_commit.hpp:17-67, 81-235
+//   * per-batch demand update            bit_util.hpp:75-90
+//
+// Canonical arithmetic (the reference is nondeterministic here; these choices
+// make every output reproducible and are shared with the HIP path):
+//   * cell cost c = LUT[clamp(cap - dem, -256, 255)], LUT entry = fp32 rounding
+//     of the model evaluated in double, saturated at 1e30;
+//   * Fenwick node covering an aligned block = pairwise fp32 sum of its cells,
+//     recomputed from integer demand after every batch (the reference builds
+//     and updates nodes with float atomicAdd, bit_util.hpp:44,51);
+//   * prefix query Q(x) accumulates nodes in the reference's descent order
+//     (res = 0; res += node[i]; i -= i & -i), bit_util.hpp:19-35;
+//   * ties: lowest pattern p, lowest pair index o, candidates in ascending root.
+//
+// Pinning of each function is listed in tests/test_oracle.py (known-answer
+// tests, brute-force cell walks in double, exhaustive small cases).
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+constexpr int kRMin = -256;
+constexpr int kRMax = 255;
+constexpr int kLut = kRMax - kRMin + 1;
+constexpr float kSat = 1e30f;
+
+// Cost models: 0 = reference BIT f(r) = exp(-r/2) (bit_util.hpp:13-15),
+// 1 = reference incremental f(r-1) - f(r) (util.hpp:127-129),
+// 2 = literal exp(d + 1 - c) = exp(1 - r) (README.md:10).
+void build_lut(int model, float* lut) {
+  for (int i = 0; i < kLut; ++i) {
+    int r = i + kRMin;
+    float v;
+    if (model == 1) {
+      float f0 = (float)std::exp(-0.5 * (double)(r - 1));
+      float f1 = (float)std::exp(-0.5 * (double)r);
+      v = f0 - f1;
+    } else if (model == 2) {
+      v = (float)std::exp(1.0 - (double)r);
+    } else {
+      v = (float)std::exp(-0.5 * (double)r);
+    }
+    if (!(v < kSat)) v = kSat;
+    lut[i] = v;
+  }
+}
+
+inline uint32_t ord_key(float f) {
+  uint32_t u;
+  f = f + 0.0f;  // -0 -> +0 so that -0 and +0 compare equal as in float ==
+  std::memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+struct Ctx {
+  int X, Y, Z, Z2, model, commit_mode;
+  int64_t cs;
+  std::vector<int32_t> cap0, rem, stamp;
+  std::vector<float> pfx;  // materialised Q(x) per cell, WIRE_ENC order
+  float lut[kLut];
+  int32_t bstamp = 0;
+  // nets
+  std::vector<int64_t> pin_pfx;
+  std::vector<int32_t> pins;
+  // last batch pair results
+  std::vector<float> last_cost;
+  std::vector<int16_t> last_pat;
+
+  // util.hpp:38-44
+  inline int64_t enc(int z, int x, int y) const {
+    return (z & 1) ? (int64_t)z * X * Y + (int64_t)x * Y + y
+                   : (int64_t)z * X * Y + (int64_t)y * X + x;
+  }
+  // util.hpp:55-59
+  inline void dec(int64_t v, int& z, int& x, int& y) const {
+    z = (int)(v / ((int64_t)X * Y));
+    if (z & 1) { x = (int)(v / Y % X); y = (int)(v % Y); }
+    else       { x = (int)(v % X);     y = (int)(v / X % Y); }
+  }
+  inline int run_len(int z) const { return (z & 1) ? Y : X; }
+  // Q(z, line, pos): sum of cells 0..pos on the run, pos = -1 gives 0.
+  inline float Q(int z, int line, int pos) const {
+    if (pos < 0) return 0.0f;
+    int L = run_len(z);
+    int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
+    return pfx[base + pos];
+  }
+  inline float cost_of_rem(int32_t r) const {
+    int i = r < kRMin ? kRMin : (r > kRMax ? kRMax : r);
+    return lut[i - kRMin];
+  }
+
+  // Rebuild the materialised prefix of one run (z, line) from integer state.
+  void refresh_run(int z, int line) {
+    int L = run_len(z);
+    int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
+    std::vector<std::vector<float>> lv;
+    lv.emplace_back(L);
+    for (int i = 0; i < L; ++i) lv[0][i] = cost_of_rem(rem[base + i]);
+    for (int k = 1; (L >> k) > 0; ++k) {
+      int n = L >> k;
+      lv.emplace_back(n);
+      for (int j = 0; j < n; ++j) lv[k][j] = lv[k - 1][2 * j] + lv[k - 1][2 * j + 1];
+    }
+    for (int x = 0; x < L; ++x) {
+      // bit_util.hpp:19-35: res = 0; while (i > 0) { res += node[i]; i -= i & -i; }
+      uint32_t i = (uint32_t)x + 1;
+      float res = 0.0f;
+      while (i > 0) {
+        int k = __builtin_ctz(i);
+        res += lv[k][(i >> k) - 1];
+        i &= i - 1;
+      }
+      pfx[base + x] = res;
+    }
+  }
+  void refresh_all() {
+    for (int z = 0; z < Z; ++z) {
+      int lines = (z & 1) ? X : Y;
+      for (int l = 0; l < lines; ++l) refresh_run(z, l);
+    }
+  }
+
+  // bit_util.hpp:108-117: each via layer i in [min, max) adds Q(pos) - Q(pos-1).
+  inline void via(int za, int zb, int x, int y, float& cost) const {
+    int lo = std::min(za, zb), hi = std::max(za, zb);
+    for (int i = lo; i < hi; ++i) {
+      if (i & 1) cost += Q(i, x, y) - Q(i, x, y - 1);
+      else       cost += Q(i, y, x) - Q(i, y, x - 1);
+    }
+  }
+  // bit_util.hpp:91-107: inclusive wire along layer z's direction.
+  inline void wire(int z, int& xs, int& ys, int x2, int y2, float& cost) const {
+    if (z & 1) {
+      if (ys < y2)      cost += Q(z, xs, y2) - Q(z, xs, ys - 1);
+      else if (ys > y2) cost += Q(z, xs, ys) - Q(z, xs, y2 - 1);
+      ys = y2;
+    } else {
+      if (xs < x2)      cost += Q(z, ys, x2) - Q(z, ys, xs - 1);
+      else if (xs > x2) cost += Q(z, ys, xs) - Q(z, ys, x2 - 1);
+      xs = x2;
+    }
+  }
+  // bit_util.hpp:131-179 for one (pair, pattern).
+  float pattern_cost(int64_t src, int64_t dst, int p, bool& valid) const {
+    int z1, x1, y1, z2, x2, y2;
+    dec(src, z1, x1, y1);
+    dec(dst, z2, x2, y2);
+    valid = false;
+    int a = p / Z2, b = p % Z2;
+    if (a == 0) return 0.0f;
+    bool D = (x1 != x2 && y1 != y2);
+    int zs = z1, xs = x1, ys = y1;
+    float cost = 0.0f;
+    if (zs != a) { via(zs, a, xs, ys, cost); zs = a; }
+    wire(zs, xs, ys, x2, y2, cost);
+    int bp = (zs & 1) ? 2 * b : 2 * b + 1;
+    if (D && bp != 0) {
+      if (bp >= Z) return 0.0f;  // b' < Z enforced (the reference reads past the top layer)
+      via(zs, bp, xs, ys, cost);
+      zs = bp;
+      wire(zs, xs, ys, x2, y2, cost);
+    }
+    if (zs != z2) { via(zs, z2, xs, ys, cost); zs = z2; }
+    if (xs != x2 || ys != y2) return 0.0f;
+    valid = true;
+    return cost;
+  }
+  // Minimum over p of (cost, p), ascending p with strict <, lpattern.hpp:144-157.
+  void best_pattern(int64_t src, int64_t dst, float& best, int& bp) const {
+    bool have = false;
+    best = 0.0f; bp = -1;
+    for (int p = 0; p < Z * Z2; ++p) {
+      bool v;
+      float c = pattern_cost(src, dst, p, v);
+      if (!v) continue;
+      if (!have || ord_key(c) < ord_key(best)) { best = c; bp = p; have = true; }
+    }
+  }
+
+  // mst_bit_commit.hpp:36-67,147-235: cells marked for one committed edge.
+  void edge_cells(int64_t src, int64_t dst, int p, std::vector<int64_t>& out) const {
+    int z1, x1, y1, z2, x2, y2;
+    dec(src, z1, x1, y1);
+    dec(dst, z2, x2, y2);
+    int a = p / Z2, b = p % Z2;
+    bool D = (x1 != x2 && y1 != y2);
+    int z = z1, x = x1, y = y1;
+    int k = 0;
+    auto interior = [&](int za, int zb) {
+      int lo = std::min(za, zb), hi = std::max(za, zb);
+      for (int l = lo + 1; l < hi; ++l) out.push_back(enc(l, x, y));
+      return hi - lo - 1;
+    };
+    auto wire_cells = [&](int zw, bool first) {
+      if (zw & 1) {
+        int lo = std::min(y, y2), hi = std::max(y, y2);
+        for (int t = lo; t <= hi; ++t) out.push_back(enc(zw, x, t));
+        if (first && commit_mode == 0)
+          for (int t = 1; t <= k; ++t) out.push_back(enc(zw, x, lo) - t);
+        y = y2;
+      } else {
+        int lo = std::min(x, x2), hi = std::max(x, x2);
+        for (int t = lo; t <= hi; ++t) out.push_back(enc(zw, t, y));
+        if (first && commit_mode == 0)
+          for (int t = 1; t <= k; ++t) out.push_back(enc(zw, lo, y) - t);
+        x = x2;
+      }
+    };
+    if (z != a) { k = interior(z, a); z = a; }
+    wire_cells(z, true);
+    int bp = (z & 1) ? 2 * b : 2 * b + 1;
+    if (D && bp != 0) {
+      interior(z, bp);
+      z = bp;
+      wire_cells(z, false);
+    }
+    if (z != z2) { interior(z, z2); z = z2; }
+  }
+};
+
+// Root-pair Boruvka on one net, gmst_util.hpp:164-341 with the lowest-index
+// tie contract.  costs[o] for o = j(j-1)/2 + k.  Emits (on1, on2) pairs where
+// on1 is the selecting component's root at round start and on2 the partner's.
+int boruvka(int n, const float* costs, std::vector<std::pair<int, int>>& edges,
+            std::vector<int>& eo, float& net_cost) {
+  std::vector<int> dsu(n), partner(n);
+  std::vector<uint64_t> best(n);
+  for (int i = 0; i < n; ++i) dsu[i] = i;
+  auto find = [&](int v) {
+    int r = v;
+    while (dsu[r] != r) r = dsu[r];
+    dsu[v] = r;  // gmst_util.hpp:68-82 compresses only the queried node
+    return r;
+  };
+  edges.clear(); eo.clear();
+  net_cost = 0.0f;
+  int rounds = 0;
+  while ((int)edges.size() < n - 1) {
+    ++rounds;
+    std::fill(best.begin(), best.end(), ~0ull);
+    int64_t P = (int64_t)n * (n - 1) / 2;
+    int64_t o = 0;
+    for (int j = 1; j < n; ++j) {
+      for (int k = 0; k < j; ++k, ++o) {
+        int rj = find(j), rk = find(k);
+        if (rj == rk) continue;
+        uint64_t key = ((uint64_t)ord_key(costs[o]) << 32) | (uint64_t)o;
+        if (key < best[rj]) best[rj] = key;
+        if (key < best[rk]) best[rk] = key;
+      }
+    }
+    (void)P;
+    // partners at round start (gmst_util.hpp:245-262)
+    for (int r = 0; r < n; ++r) {
+      partner[r] = -1;
+      if (dsu[r] != r || best[r] == ~0ull) continue;
+      int64_t bo = (int64_t)(best[r] & 0xffffffffu);
+      int j = (int)((1.0 + std::sqrt(1.0 + 8.0 * (double)bo)) / 2.0);
+      while ((int64_t)j * (j - 1) / 2 > bo) --j;
+      while ((int64_t)(j + 1) * j / 2 <= bo) ++j;
+      int k = (int)(bo - (int64_t)j * (j - 1) / 2);
+      int rj = find(j), rk = find(k);
+      partner[r] = (rj == r) ? rk : rj;
+    }
+    // serial union in ascending root order (gmst_util.hpp:296-341)
+    for (int r = 0; r < n; ++r) {
+      if (partner[r] < 0) continue;
+      int n1 = find(r), n2 = find(partner[r]);
+      if (n1 == n2) continue;
+      if (n1 < n2) dsu[n1] = n2; else dsu[n2] = n1;
+      net_cost += costs[best[r] & 0xffffffffu];
+      edges.emplace_back(r, partner[r]);
+      eo.push_back((int)(best[r] & 0xffffffffu));
+    }
+  }
+  return rounds;
+}
+
+inline int64_t pair_index(int hi, int lo) { return (int64_t)hi * (hi - 1) / 2 + lo; }
+
+}  // namespace
+
+extern "C" {
+
+void orc_lut(int model, float* out) { build_lut(model, out); }
+int orc_lut_rmin() { return kRMin; }
+int orc_lut_size() { return kLut; }
+
+void* orc_create(int X, int Y, int Z, int cost_model, int commit_mode, const int32_t* cap) {
+  if (X <= 0 || Y <= 0 || Z < 2 || !cap) return nullptr;
+  Ctx* c = new Ctx();
+  c->X = X; c->Y = Y; c->Z = Z; c->Z2 = (Z + 1) / 2;
+  c->model = cost_model; c->commit_mode = commit_mode;
+  c->cs = (int64_t)X * Y * Z;
+  c->cap0.assign(cap, cap + c->cs);
+  c->rem = c->cap0;
+  c->stamp.assign(c->cs, 0);
+  c->pfx.assign(c->cs, 0.0f);
+  build_lut(cost_model, c->lut);
+  c->refresh_all();
+  return c;
+}
+
+void orc_destroy(void* h) { delete (Ctx*)h; }
+
+int orc_set_nets(void* h, int n_nets, const int64_t* pin_pfx, const int32_t* pins) {
+  Ctx* c = (Ctx*)h;
+  c->pin_pfx.assign(pin_pfx, pin_pfx + n_nets + 1);
+  c->pins.assign(pins, pins + pin_pfx[n_nets]);
+  return 0;
+}
+
+int orc_get_demand(void* h, int32_t* out) {
+  Ctx* c = (Ctx*)h;
+  for (int64_t i = 0; i < c->cs; ++i) out[i] = c->cap0[i] - c->rem[i];
+  return 0;
+}
+
+int orc_get_pfx(void* h, float* out) {
+  Ctx* c = (Ctx*)h;
+  std::memcpy(out, c->pfx.data(), sizeof(float) * c->cs);
+  return 0;
+}
+
+int orc_set_rem(void* h, const int32_t* rem) {
+  Ctx* c = (Ctx*)h;
+  c->rem.assign(rem, rem + c->cs);
+  c->refresh_all();
+  return 0;
+}
+
+float orc_pattern_cost(void* h, int32_t src, int32_t dst, int p, int* valid) {
+  bool v;
+  float r = ((Ctx*)h)->pattern_cost(src, dst, p, v);
+  *valid = v;
+  return r;
+}
+
+// Brute-force restatement of one pattern's cost: walk every cell of the
+// multiset of A.4 explicitly and sum the cell costs in double.  Used only to
+// pin the Fenwick evaluation (the two agree up to fp32 rounding).
+double orc_pattern_cost_walk(void* h, int32_t src, int32_t dst, int p, int* valid) {
+  Ctx* c = (Ctx*)h;
+  int z1, x1, y1, z2, x2, y2;
+  c->dec(src, z1, x1, y1);
+  c->dec(dst, z2, x2, y2);
+  *valid = 0;
+  int a = p / c->Z2, b = p % c->Z2;
+  if (a == 0) return 0.0;
+  bool D = (x1 != x2 && y1 != y2);
+  int z = z1, x = x1, y = y1;
+  double s = 0.0;
+  auto cell = [&](int zz, int xx, int yy) { return (double)c->cost_of_rem(c->rem[c->enc(zz, xx, yy)]); };
+  auto via = [&](int za, int zb) {
+    for (int l = std::min(za, zb); l < std::max(za, zb); ++l) s += cell(l, x, y);
+  };
+  auto wire = [&](int zw) {
+    if (zw & 1) { if (y != y2) for (int t = std::min(y, y2); t <= std::max(y, y2); ++t) s += cell(zw, x, t); y = y2; }
+    else        { if (x != x2) for (int t = std::min(x, x2); t <= std::max(x, x2); ++t) s += cell(zw, t, y); x = x2; }
+  };
+  if (z != a) { via(z, a); z = a; }
+  wire(z);
+  int bp = (z & 1) ? 2 * b : 2 * b + 1;
+  if (D && bp != 0) {
+    if (bp >= c->Z) return 0.0;
+    via(z, bp); z = bp; wire(z);
+  }
+  if (z != z2) { via(z, z2); z = z2; }
+  if (x != x2 || y != y2) return 0.0;
+  *valid = 1;
+  return s;
+}
+
+int orc_best_pattern(void* h, int32_t src, int32_t dst, float* cost) {
+  float b; int p;
+  ((Ctx*)h)->best_pattern(src, dst, b, p);
+  *cost = b;
+  return p;
+}
+
+int orc_edge_cells(void* h, int32_t src, int32_t dst, int p, int64_t* out, int cap) {
+  std::vector<int64_t> v;
+  ((Ctx*)h)->edge_cells(src, dst, p, v);
+  int n = (int)std::min<size_t>(v.size(), (size_t)cap);
+  for (int i = 0; i < n; ++i) out[i] = v[i];
+  return (int)v.size();
+}
+
+// Boruvka on an explicit cost array (local net 0).  out_edges gets n slots:
+// n-1 encoded edges then -1; returns the number of rounds.
+int orc_boruvka(int n, const float* costs, int64_t* out_edges, float* out_cost) {
+  std::vector<std::pair<int, int>> e;
+  std::vector<int> eo;
+  float nc;
+  int rounds = boruvka(n, costs, e, eo, nc);
+  for (size_t i = 0; i < e.size(); ++i)
+    out_edges[i] = ((int64_t)e[i].first << 15) | (int64_t)e[i].second;
+  out_edges[n - 1] = -1;
+  *out_cost = nc;
+  return rounds;
+}
+
+// One batch of nets (disjoint bounding boxes).  Slot layout: net i of the
+// batch owns deg(i) consecutive slots; n-1 edges then a -1 sentinel
+// (gmst_util.hpp:322-339).  out_lpat gets each edge's pattern, -1 on the
+// sentinel.  Returns the number of distinct cells whose demand was bumped.
+int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edges,
+                        int32_t* out_lpat, float* out_net_cost, int nthreads) {
+  Ctx* c = (Ctx*)h;
+  std::vector<int64_t> slot_pfx(n + 1, 0), pair_pfx(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int64_t d = c->pin_pfx[net_ids[i] + 1] - c->pin_pfx[net_ids[i]];
+    slot_pfx[i + 1] = slot_pfx[i] + d;
+    pair_pfx[i + 1] = pair_pfx[i] + d * (d - 1) / 2;
+  }
+  c->last_cost.assign(pair_pfx[n], 0.0f);
+  c->last_pat.assign(pair_pfx[n], -1);
+  std::vector<std::vector<int64_t>> cells(n);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    const int32_t* pins = c->pins.data() + c->pin_pfx[net_ids[i]];
+    int d = (int)(slot_pfx[i + 1] - slot_pfx[i]);
+    float* pc = c->last_cost.data() + pair_pfx[i];
+    int16_t* pp = c->last_pat.data() + pair_pfx[i];
+    int64_t o = 0;
+    for (int j = 1; j < d; ++j)
+      for (int k = 0; k < j; ++k, ++o) {
+        float b; int p;
+        c->best_pattern(pins[j], pins[k], b, p);
+        pc[o] = b; pp[o] = (int16_t)p;
+      }
+    std::vector<std::pair<int, int>> e;
+    std::vector<int> eo;
+    float nc;
+    boruvka(d, pc, e, eo, nc);
+    out_net_cost[i] = nc;
+    int64_t base = slot_pfx[i];
+    for (size_t t = 0; t < e.size(); ++t) {
+      int on1 = e[t].first, on2 = e[t].second;
+      out_edges[base + t] = ((int64_t)i << 30) | ((int64_t)on1 << 15) | (int64_t)on2;
+      int hi = std::max(on1, on2), lo = std::min(on1, on2);
+      int p = pp[pair_index(hi, lo)];
+      out_lpat[base + t] = p;
+      c->edge_cells(pins[hi], pins[lo], p, cells[i]);
+    }
+    out_edges[base + d - 1] = -1;
+    out_lpat[base + d - 1] = -1;
+  }
+  // bit_util.hpp:75-90: each flagged cell loses one unit of capacity once.
+  c->bstamp++;
+  int64_t touched = 0;
+  std::vector<char> dirty((size_t)c->Z * std::max(c->X, c->Y), 0);
+  for (int i = 0; i < n; ++i)
+    for (int64_t cell : cells[i]) {
+      if (c->stamp[cell] == c->bstamp) continue;
+      c->stamp[cell] = c->bstamp;
+      c->rem[cell] -= 1;
+      ++touched;
+      int z, x, y;
+      c->dec(cell, z, x, y);
+      int line = (z & 1) ? x : y;
+      dirty[(size_t)z * std::max(c->X, c->Y) + line] = 1;
+    }
+  for (int z = 0; z < c->Z; ++z) {
+    int lines = (z & 1) ? c->X : c->Y;
+    for (int l = 0; l < lines; ++l)
+      if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) c->refresh_run(z, l);
+  }
+  return touched;
+}
+
+int64_t orc_last_pairs(void* h) { return (int64_t)((Ctx*)h)->last_cost.size(); }
+
+int orc_get_pair_costs(void* h, float* cost, int16_t* pat) {
+  Ctx* c = (Ctx*)h;
+  std::memcpy(cost, c->last_cost.data(), sizeof(float) * c->last_cost.size());
+  std::memcpy(pat, c->last_pat.data(), sizeof(int16_t) * c->last_pat.size());
+  return 0;
+}
+
+}  // extern "C"
