@@ -1,0 +1,163 @@
+"""Benchmark: nets routed per second on one MI355X.
+
+A step is one full routing pass over the workload: reset demand, then route
+every batch of the schedule in order (L-pattern costing -> root-pair Boruvka
+-> Fenwick commit -> cost refresh per batch).  The default workload is the
+800x800x10 GCELL grid with 150,000 nets.  Inputs are resident in HBM before
+the timed region.  Per-kernel durations come from HIP events recorded on the
+library's launch stream during the timed steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "nets routed/sec on 1 MI355X, learned+batched vs plain, at ≤0.5% route-cost gap"
+HBM_PEAK_GBS = 8000.0
+KNAMES = ["k_lpat", "k_tree", "k_refresh"]
+
+
+def footprint(X, Y, Z, pfx, pins):
+    """Per-net pin count n, pair count P and unique-cell footprint F of the L-pattern stage."""
+    n_nets = len(pfx) - 1
+    deg = np.diff(pfx)
+    net = np.repeat(np.arange(n_nets, dtype=np.int64), deg)
+    p = pins.astype(np.int64)
+    z = p // (X * Y)
+    r = p - z * X * Y
+    odd = (z & 1) == 1
+    x = np.where(odd, r // Y, r % X)
+    y = np.where(odd, r % Y, r // X)
+    xs = np.bincount(net[np.unique(net * X + x, return_index=True)[1]], minlength=n_nets)
+    ys = np.bincount(net[np.unique(net * Y + y, return_index=True)[1]], minlength=n_nets)
+    w = np.zeros(n_nets, np.int64)
+    h = np.zeros(n_nets, np.int64)
+    np.maximum.at(w, net, x); np.maximum.at(h, net, y)
+    wmin = np.full(n_nets, 1 << 40, np.int64); hmin = np.full(n_nets, 1 << 40, np.int64)
+    np.minimum.at(wmin, net, x); np.minimum.at(hmin, net, y)
+    W = w - wmin + 1
+    H = h - hmin + 1
+    E = (Z - 1) // 2
+    O = Z // 2
+    F = Z * xs * ys + E * ys * (W - xs) + O * xs * (H - ys)
+    P = deg * (deg - 1) // 2
+    return deg, P, F
+
+
+def cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, budget_s):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    o = oracle.Oracle(X, Y, Z, cap)
+    o.set_nets(pfx, pins)
+    nets = 0
+    t0 = time.perf_counter()
+    b = 0
+    while b < len(ofst) - 1:
+        batch = ids[ofst[b]:ofst[b + 1]]
+        o.route_batch(batch, nthreads=threads)
+        nets += len(batch)
+        b += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": nets / dt, "unit": "nets/s", "cores": threads, "kind": "port",
+            "sample": f"batches 0..{b - 1} of {len(ofst) - 1} ({nets} nets) of the same workload, "
+                      f"C++ oracle with OpenMP over nets, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the routing path runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import iitmglobalrouter_amd as gr
+    W = gr.WORKLOADS[args.workload]
+    X, Y, Z = W["X"], W["Y"], W["Z"]
+    t = time.perf_counter()
+    cap = gr.gen_grid(args.workload, args.seed, X, Y, Z)
+    pfx, pins = gr.gen_nets(args.workload, args.seed, X, Y, Z, W["nets"])
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    print(f"# setup {time.perf_counter() - t:.2f}s nets {len(pfx) - 1} batches {len(ofst) - 1}",
+          file=sys.stderr, flush=True)
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    n_nets = len(pfx) - 1
+    for _ in range(args.warmup):
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+    print("# warmup done", file=sys.stderr, flush=True)
+    if not args.no_timing:
+        r.enable_timing(True)
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset_demand()
+        r.route_schedule()
+    r.sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n_nets / ((t1 - t0) / args.steps)
+    print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    deg, P, F = footprint(X, Y, Z, pfx, pins)
+    G = int(r.demand().sum())  # committed cells over the pass = total demand bumps
+    bytes_lpat = float((4 * deg + 4 * F + 6 * P).sum())
+    bytes_tree = float((4 * P + 12 * deg).sum() + 8 * G)
+    bytes_path = float((4 * deg + 4 * F + 10 * (deg - 1)).sum() + 8 * G)
+    roof = None
+    kstats = {}
+    if not args.no_timing:
+        n, ms = r.kernel_stats()
+        n_b = len(ofst) - 1
+        for k in range(3):
+            kstats[KNAMES[k]] = {"launches": int(n[k]), "ms_per_step": float(ms[k]) / args.steps}
+        dom = int(np.argmax(ms))
+        per_launch_bytes = {0: bytes_lpat / n_b, 1: bytes_tree / n_b}.get(dom)
+        if per_launch_bytes is not None:
+            avg_s = ms[dom] / 1e3 / n[dom]
+            ach = per_launch_bytes / avg_s / 1e9
+            roof = {"kernel": KNAMES[dom], "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                    "avg_launch_us": round(avg_s * 1e6, 2)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, args.cpu_seconds)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "nets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain GPU path; {X}x{Y}x{Z} GCELLs, {W['nets']} nets requested "
+                               f"(seed {args.seed}), {n_nets} routed after pin dedupe, "
+                               f"{len(ofst) - 1} batches from 4 x-buckets",
+                   "grid": [X, Y, Z], "nets": n_nets, "pairs": int(P.sum()), "batches": len(ofst) - 1,
+                   "mode": "plain"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kstats,
+        "path_bytes_per_step": bytes_path,
+        "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

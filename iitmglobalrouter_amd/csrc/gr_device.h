@@ -97,7 +97,7 @@ __device__ __forceinline__ bool pattern_cost(const float* __restrict__ pfx, cons
 
 // Pair index o = j(j-1)/2 + k, k < j (test_lpat.hpp:316-331) -> (j, k).
 __device__ __forceinline__ void pair_decode(uint32_t o, int& j, int& k) {
-  int jj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)o)) * 0.5);
+  int jj = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)o)) * 0.5f);
   while ((uint64_t)jj * (jj - 1) / 2 > o) --jj;
   while ((uint64_t)(jj + 1) * jj / 2 <= o) ++jj;
   j = jj;

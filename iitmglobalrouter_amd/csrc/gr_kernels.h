@@ -1,0 +1,533 @@
+// Kernels of the routing hot path (gfx950, wave64).
+//
+// Device state (WIRE_ENC cell order, util.hpp:38-44; every layer is a set of
+// contiguous runs: rows of length X on even layers, columns of length Y on odd):
+//   rem[cell]        int32  remaining capacity cap - demand (the integer state)
+//   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
+//                           cell, so a reference range query Q(hi) - Q(lo-1)
+//                           (bit_util.hpp:36-38) costs two loads
+//   stamp[cell]      int32  batch stamp of the last commit that used the cell
+//   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
+// Per batch:
+//   k_lpat_lane  one lane per pin pair evaluates every L-pattern serially
+//   k_tree2/_small/_large  root-pair Boruvka + commit: one lane per 2-pin net,
+//                one wavefront per net up to 64 pins, one workgroup above
+//   k_refresh    per dirty run: demand -1 on stamped cells, canonical Fenwick
+//                nodes rebuilt in LDS, run's pfx rewritten
+#pragma once
+#include "gr_device.h"
+
+namespace grd {
+
+// ---------------------------------------------------------------------------
+// Cost refresh
+// ---------------------------------------------------------------------------
+
+// Runs are scanned by a persistent grid; run id r = z*M + line.  all != 0
+// rebuilds every run without touching demand (set-up and reset).  Otherwise a
+// run is processed only when the batch's commit stamped it: its stamped cells
+// lose one unit of capacity (bit_util.hpp:75-90 applies cap -= 1 once per
+// flagged cell), then Fenwick node i (1-based, lowbit 2^k, covering cells
+// [i - 2^k, i)) is the pairwise fp32 sum of its aligned block, built level by
+// level in LDS, and Q(x) accumulates nodes in the reference's descent order
+// (bit_util.hpp:19-35).
+__global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
+                                                 const int32_t* __restrict__ stamp, float* __restrict__ pfx,
+                                                 const float* __restrict__ lut,
+                                                 const int32_t* __restrict__ run_stamp, int32_t bstamp, int all) {
+  extern __shared__ float sh[];
+  float* slut = sh;
+  float* lv = sh + kLut;
+  __shared__ int offk[16];
+  for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
+  __syncthreads();
+  const int nr = g.Z * M;
+  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    const int z = r / M, line = r - z * M;
+    const int lines = (z & 1) ? g.X : g.Y;
+    if (line >= lines) continue;
+    if (!all && run_stamp[r] != bstamp) continue;
+    const int L = (z & 1) ? g.Y : g.X;
+    const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+      int v = rem[base + i];
+      if (!all && stamp[base + i] == bstamp) {
+        v -= 1;
+        rem[base + i] = v;
+      }
+      v = v < kRMin ? kRMin : (v > kRMin + kLut - 1 ? kRMin + kLut - 1 : v);
+      lv[i] = slut[v - kRMin];
+    }
+    if (threadIdx.x == 0) {
+      int off = 0;
+      for (int k = 0; (L >> k) > 0; ++k) { offk[k] = off; off += L >> k; }
+    }
+    __syncthreads();
+    for (int k = 1; (L >> k) > 0; ++k) {
+      const int nk = L >> k, o = offk[k], op = offk[k - 1];
+      for (int j = threadIdx.x; j < nk; j += blockDim.x) lv[o + j] = lv[op + 2 * j] + lv[op + 2 * j + 1];
+      __syncthreads();
+    }
+    for (int x = threadIdx.x; x < L; x += blockDim.x) {
+      uint32_t i = (uint32_t)x + 1;
+      float res = 0.0f;
+      while (i) {
+        const int k = __builtin_ctz(i);
+        res += lv[offk[k] + (int)(i >> k) - 1];
+        i &= i - 1;
+      }
+      pfx[base + x] = res;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// L-pattern costing
+// ---------------------------------------------------------------------------
+
+// Single-cell via term at (x, y) on layer i: Q(pos) - Q(pos - 1), bit_util.hpp:108-117.
+__device__ __forceinline__ float dterm(const float* __restrict__ pfx, const Geo& g, int i, int x, int y) {
+  return (i & 1) ? Q(pfx, g, i, x, y) - Q(pfx, g, i, x, y - 1) : Q(pfx, g, i, y, x) - Q(pfx, g, i, y, x - 1);
+}
+// Inclusive wire u..v on a run: Q(hi) - Q(lo - 1), bit_util.hpp:91-107.
+__device__ __forceinline__ float wterm(const float* __restrict__ pfx, const Geo& g, int z, int line, int u, int v) {
+  const int lo = u < v ? u : v, hi = u < v ? v : u;
+  return Q(pfx, g, z, line, hi) - Q(pfx, g, z, line, lo - 1);
+}
+
+// Pair w of the batch -> (batch position, local pair index).  chunk_net[c]
+// holds the position of the net that owns pair 64*c; a net has >= 1 pair, so
+// the owner of w is among the next 64 positions.
+__device__ __forceinline__ int pair_owner(const int64_t* __restrict__ bpair, const int32_t* __restrict__ chunk_net,
+                                          int n, int64_t w) {
+  int lo = chunk_net[w >> 6];
+  int hi = lo + 65 < n ? lo + 65 : n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bpair[mid] <= w) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One lane per pin pair.  All via and wire terms of the pair are loaded once
+// into registers; patterns are then evaluated in ascending p with exactly the
+// oracle's fp32 operation sequence (bit_util.hpp:131-179):
+//   cost = 0; += via cells z1..a (ascending); += wire on a;
+//   [D and b' != 0: += via cells a..b' at the corner (ascending); += wire on b'];
+//   += via cells to z2 at the destination (ascending).
+// The source via sum for a > z1 is a running sum shared across a.  For
+// straight pairs (not D) all b tie, so only b = 0 is evaluated.  Selection is
+// strict-less in ascending p, i.e. the lexicographic minimum of (cost, p).
+template <int ZT>
+__global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
+                                                   const int64_t* __restrict__ pin_pfx,
+                                                   const int32_t* __restrict__ pins,
+                                                   const int32_t* __restrict__ bnet,
+                                                   const int64_t* __restrict__ bpair,
+                                                   const int32_t* __restrict__ chunk_net, int n, int64_t P,
+                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat) {
+  constexpr int Z2T = (ZT + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (w >= P) return;
+  const int pos = pair_owner(bpair, chunk_net, n, w);
+  int j, k;
+  pair_decode((uint32_t)(w - bpair[pos]), j, k);
+  const int64_t pb = pin_pfx[bnet[pos]];
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
+  dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
+  const bool D = (x1 != x2) && (y1 != y2);
+
+  float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
+#pragma unroll
+  for (int i = 0; i < ZT - 1; ++i) {
+    ds[i] = dterm(pfx, g, i, x1, y1);
+    dd[i] = dterm(pfx, g, i, x2, y2);
+  }
+#pragma unroll
+  for (int l = 1; l < ZT; ++l) {
+    // first wire on layer l from the source: rows y1 on even layers, columns x1 on odd
+    w1[l] = (l & 1) ? (y1 != y2 ? wterm(pfx, g, l, x1, y1, y2) : 0.f) : (x1 != x2 ? wterm(pfx, g, l, y1, x1, x2) : 0.f);
+  }
+  if (D) {
+#pragma unroll
+    for (int i = 0; i < ZT - 1; ++i) {
+      dc1[i] = dterm(pfx, g, i, x2, y1);  // corner after a horizontal first wire
+      dc2[i] = dterm(pfx, g, i, x1, y2);  // corner after a vertical first wire
+    }
+#pragma unroll
+    for (int l = 1; l < ZT; ++l) {
+      // second wire into the destination: columns x2 on odd layers, rows y2 on even
+      w2[l] = (l & 1) ? wterm(pfx, g, l, x2, y1, y2) : wterm(pfx, g, l, y2, x1, x2);
+    }
+  }
+
+  uint64_t best = ~0ull;
+  float vs = 0.f;  // running source via sum for a > z1
+#pragma unroll
+  for (int a = 1; a < ZT; ++a) {
+    float ca = 0.f;
+    if (a > z1) {
+      vs = (a == z1 + 1) ? ds[a - 1] : vs + ds[a - 1];
+      ca = vs;
+    } else if (a < z1) {
+#pragma unroll
+      for (int i = 0; i < ZT - 1; ++i)
+        if (i >= a && i < z1) ca += ds[i];
+    }
+    const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
+    if (moved1) ca += w1[a];
+#pragma unroll
+    for (int b = 0; b < Z2T; ++b) {
+      if (!D && b > 0) continue;
+      const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+      float c = ca;
+      int zs = a;
+      if (D && bp != 0) {
+        if (bp >= ZT) continue;
+        const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
+#pragma unroll
+        for (int i = lo; i < hi; ++i) c += (a & 1) ? dc2[i] : dc1[i];
+        c += w2[bp];
+        zs = bp;
+      } else {
+        const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
+        if (!at_dst) continue;
+      }
+      if (zs != z2) {
+        const int lo = zs < z2 ? zs : z2, hi = zs < z2 ? z2 : zs;
+#pragma unroll
+        for (int i = 0; i < ZT - 1; ++i)
+          if (i >= lo && i < hi) c += dd[i];
+      }
+      const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
+      best = key < best ? key : best;
+    }
+  }
+  out_cost[w] = key_cost((uint32_t)(best >> 32));
+  out_pat[w] = (int16_t)(uint32_t)best;
+}
+
+// Any Z (up to 32): one wavefront per pair, one lane per pattern.
+__global__ __launch_bounds__(256) void k_lpat_wave(Geo g, const float* __restrict__ pfx,
+                                                   const int64_t* __restrict__ pin_pfx,
+                                                   const int32_t* __restrict__ pins,
+                                                   const int32_t* __restrict__ bnet,
+                                                   const int64_t* __restrict__ bpair,
+                                                   const int32_t* __restrict__ chunk_net, int n, int64_t P,
+                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat) {
+  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= P) return;
+  const int pos = pair_owner(bpair, chunk_net, n, w);
+  int j, k;
+  pair_decode((uint32_t)(w - bpair[pos]), j, k);
+  const int64_t pb = pin_pfx[bnet[pos]];
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
+  dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
+  uint64_t best = ~0ull;
+  const int np = g.Z * g.Z2;
+  for (int p = lane; p < np; p += 64) {
+    float c;
+    if (pattern_cost(pfx, g, z1, x1, y1, z2, x2, y2, p, c)) {
+      const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)p;
+      best = key < best ? key : best;
+    }
+  }
+  best = wave_min_u64(best);
+  if (lane == 0) {
+    out_cost[w] = key_cost((uint32_t)(best >> 32));
+    out_pat[w] = (int16_t)(uint32_t)best;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tree + commit
+// ---------------------------------------------------------------------------
+
+struct TreeArgs {
+  Geo g;
+  const int32_t* net;   // batch position -> net id
+  const int64_t* slot;  // batch position -> first output slot
+  const int64_t* pair;  // batch position -> first pair of the net in the batch
+  const int32_t* list;  // batch positions handled by this launch
+  int n_list;
+  int64_t cost_base;    // out_cost index of batch position 0
+  const int64_t* pin_pfx;
+  const int32_t* pins;
+  const float* pair_cost;
+  const int16_t* pair_pat;
+  int64_t* out_edges;
+  int32_t* out_lpat;
+  float* out_cost;
+  int32_t* stamp;
+  int32_t* run_stamp;
+  int32_t bstamp;
+  int32_t quirk;
+  int32_t M;  // run_stamp row stride: max(X, Y)
+};
+
+// Commit marks are idempotent plain stores: the cell's stamp and its run's
+// stamp.  The demand update itself happens once per stamped cell in k_refresh.
+__device__ __forceinline__ void mark_run(const TreeArgs& A, int z, int line) {
+  A.run_stamp[z * A.M + line] = A.bstamp;
+}
+__device__ __forceinline__ void mark_any(const TreeArgs& A, uint32_t cell) {
+  A.stamp[cell] = A.bstamp;
+  int z, x, y;
+  dec(A.g, cell, z, x, y);
+  mark_run(A, z, (z & 1) ? x : y);
+}
+__device__ __forceinline__ void mark_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
+  const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
+  const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
+  for (int t = lo; t <= hi; ++t) A.stamp[base + t] = A.bstamp;
+  mark_run(A, z, line);
+}
+__device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int x, int y) {
+  const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
+  for (int l = lo + 1; l < hi; ++l) {
+    A.stamp[enc(A.g, l, x, y)] = A.bstamp;
+    mark_run(A, l, (l & 1) ? x : y);
+  }
+}
+
+// Cells of one committed edge (mst_bit_commit.hpp:36-67, 147-235): interior
+// via cells, inclusive wires.  With the quirk, the k threads that marked the
+// first via stack also mark the k cells just before the first wire's low end,
+// by enc arithmetic (:213-216: the first wire call is not guarded by tid >= 0).
+__device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
+  const Geo& g = A.g;
+  if (p < g.Z2 || p >= g.Z * g.Z2) return;  // no valid pattern (cannot happen for Z >= 3)
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, src, z1, x1, y1);
+  dec(g, dst, z2, x2, y2);
+  const int a = p / g.Z2, b = p - a * g.Z2;
+  const bool D = (x1 != x2) && (y1 != y2);
+  int z = z1, x = x1, y = y1, k = 0;
+  if (z != a) {
+    mark_via(A, z, a, x, y);
+    k = (z < a ? a - z : z - a) - 1;
+    z = a;
+  }
+  if (z & 1) {
+    const int lo = min(y, y2), hi = max(y, y2);
+    mark_wire(A, z, x, lo, hi);
+    if (A.quirk) for (int t = 1; t <= k; ++t) mark_any(A, enc(g, z, x, lo) - t);
+    y = y2;
+  } else {
+    const int lo = min(x, x2), hi = max(x, x2);
+    mark_wire(A, z, y, lo, hi);
+    if (A.quirk) for (int t = 1; t <= k; ++t) mark_any(A, enc(g, z, lo, y) - t);
+    x = x2;
+  }
+  const int bp = (z & 1) ? 2 * b : 2 * b + 1;
+  if (D && bp != 0) {
+    mark_via(A, z, bp, x, y);
+    z = bp;
+    if (z & 1) { mark_wire(A, z, x, min(y, y2), max(y, y2)); y = y2; }
+    else       { mark_wire(A, z, y, min(x, x2), max(x, x2)); x = x2; }
+  }
+  if (z != z2) mark_via(A, z, z2, x, y);
+}
+
+__device__ __forceinline__ void emit_edge(const TreeArgs& A, int pos, int64_t sb, int e, int on1, int on2,
+                                          const int16_t* pat, const int32_t* npins) {
+  A.out_edges[sb + e] = ((int64_t)pos << 30) | ((int64_t)on1 << 15) | (int64_t)on2;
+  const int hi = on1 > on2 ? on1 : on2, lo = on1 > on2 ? on2 : on1;
+  const int p = pat[(uint32_t)((int64_t)hi * (hi - 1) / 2 + lo)];
+  A.out_lpat[sb + e] = p;
+  commit_edge(A, (uint32_t)npins[hi], (uint32_t)npins[lo], p);
+}
+
+// 2-pin nets, one lane each: the only pair is (1, 0); Boruvka's single round
+// has the mutual candidates 0 -> 1 and 1 -> 0, and root 0 is processed first,
+// so the edge is (0, 1) and the tree cost is 0 + cost.
+__global__ __launch_bounds__(256) void k_tree2(TreeArgs A) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= A.n_list) return;
+  const int pos = A.list[idx];
+  const int64_t pb = A.pin_pfx[A.net[pos]];
+  const int64_t sb = A.slot[pos];
+  const int16_t* pat = A.pair_pat + A.pair[pos];
+  emit_edge(A, pos, sb, 0, 0, 1, pat, A.pins + pb);
+  A.out_edges[sb + 1] = -1;
+  A.out_lpat[sb + 1] = -1;
+  A.out_cost[A.cost_base + pos] = 0.0f + A.pair_cost[A.pair[pos]];
+}
+
+// Root-pair Boruvka (gmst_util.hpp:164-341) with the lowest-index tie contract,
+// one round at a time over a group of gsize threads (a wavefront or a
+// workgroup; node r is handled by thread r mod gsize):
+//   min phase    per component root, min (cost, o) over outgoing pairs
+//                (LDS 64-bit atomicMin; roots are kept flattened)
+//   partner      root of the other endpoint of that pair
+//   accept       the reference processes candidates serially in ascending
+//                root order and drops one whose roots already joined.  With
+//                distinct (cost, o) keys the candidates form a forest plus
+//                duplicated mutual pairs, so exactly the larger root of each
+//                mutual pair is dropped: the serial outcome, computed in
+//                parallel.  Accepted edges (root, partner) are written in
+//                ascending root order.
+//   merge        candidate links, pointer jumping to each tree's root, new
+//                root = max index of the merged component (what hooking the
+//                smaller root under the larger yields, gmst_util.hpp:314-325).
+// Rounds repeat until deg-1 edges (the reference's feedback loop,
+// mst_util.hpp:298-314, ends in the same state).
+template <bool WAVE>
+__device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int* root, unsigned long long* best,
+                            int* partner, int* link, int2* edges, float* ecost, int* wcnt, int* sh) {
+  auto sync = [&]() {
+    if (WAVE) wave_sync(); else __syncthreads();
+  };
+  const int net = A.net[pos];
+  const int64_t pb = A.pin_pfx[net];
+  const int n = (int)(A.pin_pfx[net + 1] - pb);
+  const float* cost = A.pair_cost + A.pair[pos];
+  const int16_t* pat = A.pair_pat + A.pair[pos];
+  const uint32_t P = (uint32_t)((int64_t)n * (n - 1) / 2);
+  const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
+  for (int i = tid; i < n; i += gsize) root[i] = i;
+  if (tid == 0) sh[0] = 0;
+  sync();
+  // Each round adds at least one edge (the globally smallest key is a mutual
+  // pick), so n rounds always suffice; the bound only guards termination.
+  for (int round = 0; round < n; ++round) {
+    const int ne = sh[0];
+    if (ne >= n - 1) break;
+    for (int i = tid; i < n; i += gsize) best[i] = ~0ull;
+    sync();
+    {
+      uint32_t o = tid;
+      int j = 0, k = 0;
+      if (o < P) pair_decode(o, j, k);
+      for (; o < P; o += gsize) {
+        const int rj = root[j], rk = root[k];
+        if (rj != rk) {
+          const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+          atomicMin(&best[rj], key);
+          atomicMin(&best[rk], key);
+        }
+        k += gsize;
+        while (k >= j) { k -= j; ++j; }
+      }
+    }
+    sync();
+    for (int r = tid; r < n; r += gsize) {
+      int pr = -1;
+      if (root[r] == r && best[r] != ~0ull) {
+        int j, k;
+        pair_decode((uint32_t)best[r], j, k);
+        pr = (root[j] == r) ? root[k] : root[j];
+      }
+      partner[r] = pr;
+    }
+    sync();
+    // accept + ordered slots
+    const int iters = (n + gsize - 1) / gsize;
+    for (int it = 0; it < iters; ++it) {
+      const int r = it * gsize + tid;
+      bool acc = false;
+      if (r < n) {
+        const int pr = partner[r];
+        acc = pr >= 0 && !(partner[pr] == r && r > pr);
+        link[r] = acc ? pr : r;
+      }
+      const unsigned long long m = __ballot(acc);
+      if (lane == 0) wcnt[it * nwv + wv] = __popcll(m);
+      sync();
+      if (acc) {
+        int off = ne;
+        for (int q = 0; q < it * nwv + wv; ++q) off += wcnt[q];
+        off += __popcll(m & ((1ull << lane) - 1ull));
+        edges[off] = make_int2(r, partner[r]);
+        ecost[off] = cost[(uint32_t)best[r]];
+      }
+    }
+    sync();
+    if (tid == 0) {
+      int tot = ne;
+      for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
+      sh[0] = tot;
+    }
+    // pointer jumping over the candidate forest (roots only); depth < 2^32
+    for (int jump = 0; jump < 32; ++jump) {
+      int changed = 0;
+      for (int r = tid; r < n; r += gsize) {
+        if (root[r] != r) continue;
+        const int l1 = link[r], l2 = link[l1];
+        if (l2 != l1) { link[r] = l2; changed = 1; }
+      }
+      bool any;
+      if (WAVE) {
+        any = __ballot(changed) != 0ull;
+      } else {
+        any = __syncthreads_or(changed) != 0;
+      }
+      if (!any) break;
+      sync();
+    }
+    sync();
+    // new root of each merged component = its largest old root (= max pin index)
+    for (int r = tid; r < n; r += gsize)
+      if (root[r] == r) partner[r] = r;
+    sync();
+    for (int r = tid; r < n; r += gsize)
+      if (root[r] == r) atomicMax(&partner[link[r]], r);
+    sync();
+    for (int i = tid; i < n; i += gsize) {
+      const int ro = root[i];
+      best[i] = (unsigned long long)partner[link[ro]];  // staged: reads finish before root[] changes
+    }
+    sync();
+    for (int i = tid; i < n; i += gsize) root[i] = (int)best[i];
+    sync();
+  }
+  const int64_t sb = A.slot[pos];
+  const int32_t* npins = A.pins + pb;
+  for (int e = tid; e < n - 1; e += gsize) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  if (tid == 0) {
+    float nc = 0.0f;
+    for (int e = 0; e < n - 1; ++e) nc += ecost[e];
+    A.out_edges[sb + n - 1] = -1;
+    A.out_lpat[sb + n - 1] = -1;
+    A.out_cost[A.cost_base + pos] = nc;
+  }
+}
+
+// One wavefront per net of 3..kSmallDeg pins, four nets per workgroup.
+__global__ __launch_bounds__(256) void k_tree_small(TreeArgs A) {
+  __shared__ int s_root[4][kSmallDeg];
+  __shared__ unsigned long long s_best[4][kSmallDeg];
+  __shared__ int s_partner[4][kSmallDeg];
+  __shared__ int s_link[4][kSmallDeg];
+  __shared__ int2 s_edges[4][kSmallDeg];
+  __shared__ float s_ecost[4][kSmallDeg];
+  __shared__ int s_wcnt[4][2];
+  __shared__ int s_sh[4][2];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * 4 + wv;
+  if (idx >= A.n_list) return;
+  tree_commit<true>(A, A.list[idx], lane, 64, s_root[wv], s_best[wv], s_partner[wv], s_link[wv], s_edges[wv],
+                    s_ecost[wv], s_wcnt[wv], s_sh[wv]);
+}
+
+// One workgroup per net of more than kSmallDeg pins; LDS sized by the largest
+// such net of the batch (cap nodes).
+__global__ __launch_bounds__(256) void k_tree_large(TreeArgs A, int cap) {
+  extern __shared__ unsigned long long shl[];
+  unsigned long long* best = shl;
+  int2* edges = (int2*)(best + cap);
+  int* root = (int*)(edges + cap);
+  int* partner = root + cap;
+  int* link = partner + cap;
+  float* ecost = (float*)(link + cap);
+  int* wcnt = (int*)(ecost + cap);  // (cap / 64 + 4) entries
+  __shared__ int s_sh[2];
+  tree_commit<false>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, edges, ecost, wcnt,
+                     s_sh);
+}
+
+}  // namespace grd
