@@ -4,8 +4,9 @@ A step is one full routing pass over the workload: reset demand, then route
 every batch of the schedule in order (L-pattern costing -> root-pair Boruvka
 -> Fenwick commit -> cost refresh per batch).  The default workload is the
 800x800x10 GCELL grid with 150,000 nets.  Inputs are resident in HBM before
-the timed region.  Per-kernel durations come from HIP events recorded on the
-library's launch stream during the timed steps.
+the timed region, which replays each pass as one captured HIP graph.
+Per-kernel durations come from HIP events recorded on the library's launch
+stream around every launch, in an instrumented repeat of the same passes.
 """
 import argparse
 import json
@@ -105,8 +106,7 @@ def main():
         r.route_schedule()
         r.sync()
     print("# warmup done", file=sys.stderr, flush=True)
-    if not args.no_timing:
-        r.enable_timing(True)
+    # timed region: the whole pass replays as one captured HIP graph per step
     r.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -120,24 +120,41 @@ def main():
 
     deg, P, F = footprint(X, Y, Z, pfx, pins)
     G = int(r.demand().sum())  # committed cells over the pass = total demand bumps
-    bytes_lpat = float((4 * deg + 4 * F + 6 * P).sum())
-    bytes_tree = float((4 * P + 12 * deg).sum() + 8 * G)
+    n_b = len(ofst) - 1
+    # algorithmic bytes per pass of each kernel (per-unit figures in DESIGN.md)
+    kbytes = {
+        0: float((4 * deg + 4 * F + 6 * P).sum()),                        # pins + cost footprint + (cost, pattern) out
+        1: float((4 * P + 4 * deg + 2 * (deg - 1) + 12 * deg + 4).sum() + 4 * G),  # costs, pins, patterns, slots, stamps
+        2: None,
+    }
     bytes_path = float((4 * deg + 4 * F + 10 * (deg - 1)).sum() + 8 * G)
     roof = None
     kstats = {}
     if not args.no_timing:
+        # per-kernel durations: HIP events on the launch stream around every
+        # launch, over the same number of passes (events cannot sit inside the
+        # captured graph, so these passes launch kernel by kernel)
+        r.enable_timing(True)
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            r.reset_demand()
+            r.route_schedule()
+        r.sync()
+        t3 = time.perf_counter()
         n, ms = r.kernel_stats()
-        n_b = len(ofst) - 1
+        r.enable_timing(False)
+        # k_refresh also counts the reset's one full rebuild per step
         for k in range(3):
             kstats[KNAMES[k]] = {"launches": int(n[k]), "ms_per_step": float(ms[k]) / args.steps}
-        dom = int(np.argmax(ms))
-        per_launch_bytes = {0: bytes_lpat / n_b, 1: bytes_tree / n_b}.get(dom)
-        if per_launch_bytes is not None:
-            avg_s = ms[dom] / 1e3 / n[dom]
-            ach = per_launch_bytes / avg_s / 1e9
-            roof = {"kernel": KNAMES[dom], "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                    "avg_launch_us": round(avg_s * 1e6, 2)}
+            if kbytes.get(k):
+                kstats[KNAMES[k]]["GBps"] = round(kbytes[k] * args.steps / (ms[k] / 1e3) / 1e9, 2)
+        kstats["instrumented_ms_per_step"] = (t3 - t2) * 1e3 / args.steps
+        dom = int(np.argmax(ms[:2]))
+        avg_s = ms[dom] / 1e3 / n[dom]
+        ach = kbytes[dom] / n_b / avg_s / 1e9
+        roof = {"kernel": KNAMES[dom], "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, args.cpu_seconds)

@@ -10,7 +10,7 @@
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
 // Per batch:
 //   k_lpat_lane  one lane per pin pair evaluates every L-pattern serially
-//   k_tree2/_small/_large  root-pair Boruvka + commit: one lane per 2-pin net,
+//   k_tree_small/_large  root-pair Boruvka + commit: one lane per 2-pin net,
 //                one wavefront per net up to 64 pins, one workgroup above
 //   k_refresh    per dirty run: demand -1 on stamped cells, canonical Fenwick
 //                nodes rebuilt in LDS, run's pfx rewritten
@@ -34,8 +34,10 @@ namespace grd {
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  const int32_t* __restrict__ stamp, float* __restrict__ pfx,
                                                  const float* __restrict__ lut,
-                                                 const int32_t* __restrict__ run_stamp, int32_t bstamp, int all) {
+                                                 const int32_t* __restrict__ run_stamp,
+                                                 const int32_t* __restrict__ ctr, int all) {
   extern __shared__ float sh[];
+  const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* lv = sh + kLut;
   __shared__ int offk[16];
@@ -96,18 +98,25 @@ __device__ __forceinline__ float wterm(const float* __restrict__ pfx, const Geo&
   return Q(pfx, g, z, line, hi) - Q(pfx, g, z, line, lo - 1);
 }
 
-// Pair w of the batch -> (batch position, local pair index).  chunk_net[c]
-// holds the position of the net that owns pair 64*c; a net has >= 1 pair, so
-// the owner of w is among the next 64 positions.
-__device__ __forceinline__ int pair_owner(const int64_t* __restrict__ bpair, const int32_t* __restrict__ chunk_net,
-                                          int n, int64_t w) {
-  int lo = chunk_net[w >> 6];
-  int hi = lo + 65 < n ? lo + 65 : n;
-  while (hi - lo > 1) {
+// Pair w of the batch -> batch position of the net that owns it.  chunk_net[c]
+// is the owner of pair 64*c; a net has >= 1 pair, so the owner of any pair of
+// chunk c is among the 64 positions starting there.  The wavefront of chunk c
+// loads that window of pair offsets once (coalesced) and each lane binary
+// searches it through lane shuffles.  Every lane of the wave must call this.
+__device__ __forceinline__ int pair_owner_wave(const int64_t* __restrict__ bpair,
+                                               const int32_t* __restrict__ chunk_net, int n, int64_t w) {
+  const int lane = threadIdx.x & 63;
+  const int base = chunk_net[w >> 6];
+  const int64_t v = (base + lane < n) ? bpair[base + lane] : INT64_MAX;
+  int lo = 0, hi = 64;  // v[lo] <= w < v[hi]
+#pragma unroll
+  for (int it = 0; it < 6; ++it) {
     const int mid = (lo + hi) >> 1;
-    if (bpair[mid] <= w) lo = mid; else hi = mid;
+    const int64_t vm = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(v >> 32), mid, 64) << 32) |
+                                 (uint32_t)__shfl((int)v, mid, 64));
+    if (vm <= w) lo = mid; else hi = mid;
   }
-  return lo;
+  return base + lo;
 }
 
 // One lane per pin pair.  All via and wire terms of the pair are loaded once
@@ -121,19 +130,21 @@ __device__ __forceinline__ int pair_owner(const int64_t* __restrict__ bpair, con
 // strict-less in ascending p, i.e. the lexicographic minimum of (cost, p).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
-                                                   const int64_t* __restrict__ pin_pfx,
                                                    const int32_t* __restrict__ pins,
-                                                   const int32_t* __restrict__ bnet,
+                                                   const int64_t* __restrict__ pinofs,
                                                    const int64_t* __restrict__ bpair,
                                                    const int32_t* __restrict__ chunk_net, int n, int64_t P,
-                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat) {
+                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
+                                                   int32_t* __restrict__ ctr) {
   constexpr int Z2T = (ZT + 1) / 2;
-  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (w >= P) return;
-  const int pos = pair_owner(bpair, chunk_net, n, w);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
+  const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
+  const int pos = pair_owner_wave(bpair, chunk_net, n, w);
+  if (w0 >= P) return;
   int j, k;
   pair_decode((uint32_t)(w - bpair[pos]), j, k);
-  const int64_t pb = pin_pfx[bnet[pos]];
+  const int64_t pb = pinofs[pos];
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
@@ -211,19 +222,21 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
 
 // Any Z (up to 32): one wavefront per pair, one lane per pattern.
 __global__ __launch_bounds__(256) void k_lpat_wave(Geo g, const float* __restrict__ pfx,
-                                                   const int64_t* __restrict__ pin_pfx,
                                                    const int32_t* __restrict__ pins,
-                                                   const int32_t* __restrict__ bnet,
+                                                   const int64_t* __restrict__ pinofs,
                                                    const int64_t* __restrict__ bpair,
                                                    const int32_t* __restrict__ chunk_net, int n, int64_t P,
-                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat) {
+                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
+                                                   int32_t* __restrict__ ctr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
   const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= P) return;
-  const int pos = pair_owner(bpair, chunk_net, n, w);
+  int pos = chunk_net[w >> 6];
+  while (pos + 1 < n && bpair[pos + 1] <= w) ++pos;
   int j, k;
   pair_decode((uint32_t)(w - bpair[pos]), j, k);
-  const int64_t pb = pin_pfx[bnet[pos]];
+  const int64_t pb = pinofs[pos];
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
@@ -249,13 +262,13 @@ __global__ __launch_bounds__(256) void k_lpat_wave(Geo g, const float* __restric
 
 struct TreeArgs {
   Geo g;
-  const int32_t* net;   // batch position -> net id
+  const int64_t* pinofs;  // batch position -> first pin of the net in pins[]
+  const int32_t* deg;     // batch position -> pin count
   const int64_t* slot;  // batch position -> first output slot
   const int64_t* pair;  // batch position -> first pair of the net in the batch
   const int32_t* list;  // batch positions handled by this launch
   int n_list;
   int64_t cost_base;    // out_cost index of batch position 0
-  const int64_t* pin_pfx;
   const int32_t* pins;
   const float* pair_cost;
   const int16_t* pair_pat;
@@ -264,7 +277,8 @@ struct TreeArgs {
   float* out_cost;
   int32_t* stamp;
   int32_t* run_stamp;
-  int32_t bstamp;
+  const int32_t* ctr;  // device batch counter; its value is the batch stamp
+  int32_t bstamp;      // filled from *ctr at kernel entry
   int32_t quirk;
   int32_t M;  // run_stamp row stride: max(X, Y)
 };
@@ -342,14 +356,11 @@ __device__ __forceinline__ void emit_edge(const TreeArgs& A, int pos, int64_t sb
   commit_edge(A, (uint32_t)npins[hi], (uint32_t)npins[lo], p);
 }
 
-// 2-pin nets, one lane each: the only pair is (1, 0); Boruvka's single round
+// 2-pin nets: the only pair is (1, 0); Boruvka's single round
 // has the mutual candidates 0 -> 1 and 1 -> 0, and root 0 is processed first,
 // so the edge is (0, 1) and the tree cost is 0 + cost.
-__global__ __launch_bounds__(256) void k_tree2(TreeArgs A) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= A.n_list) return;
-  const int pos = A.list[idx];
-  const int64_t pb = A.pin_pfx[A.net[pos]];
+__device__ void tree2_commit(const TreeArgs& A, int pos) {
+  const int64_t pb = A.pinofs[pos];
   const int64_t sb = A.slot[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
   emit_edge(A, pos, sb, 0, 0, 1, pat, A.pins + pb);
@@ -378,17 +389,23 @@ __global__ __launch_bounds__(256) void k_tree2(TreeArgs A) {
 // mst_util.hpp:298-314, ends in the same state).
 template <bool WAVE>
 __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int* root, unsigned long long* best,
-                            int* partner, int* link, int2* edges, float* ecost, int* wcnt, int* sh) {
+                            int* partner, int* link, int2* edges, float* ecost, int* wcnt, int* sh, float* scost,
+                            int scap) {
   auto sync = [&]() {
     if (WAVE) wave_sync(); else __syncthreads();
   };
-  const int net = A.net[pos];
-  const int64_t pb = A.pin_pfx[net];
-  const int n = (int)(A.pin_pfx[net + 1] - pb);
-  const float* cost = A.pair_cost + A.pair[pos];
+  const int64_t pb = A.pinofs[pos];
+  const int n = A.deg[pos];
+  const float* gcost = A.pair_cost + A.pair[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
   const uint32_t P = (uint32_t)((int64_t)n * (n - 1) / 2);
   const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
+  // every round reads all P costs: keep them in LDS when they fit
+  const float* cost = gcost;
+  if (P <= (uint32_t)scap) {
+    for (uint32_t o = tid; o < P; o += gsize) scost[o] = gcost[o];
+    cost = scost;
+  }
   for (int i = tid; i < n; i += gsize) root[i] = i;
   if (tid == 0) sh[0] = 0;
   sync();
@@ -400,18 +417,29 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
     for (int i = tid; i < n; i += gsize) best[i] = ~0ull;
     sync();
     {
-      uint32_t o = tid;
-      int j = 0, k = 0;
-      if (o < P) pair_decode(o, j, k);
-      for (; o < P; o += gsize) {
-        const int rj = root[j], rk = root[k];
-        if (rj != rk) {
-          const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
-          atomicMin(&best[rj], key);
-          atomicMin(&best[rk], key);
+      // node-centric: S adjacent threads scan node i's n-1 pairs (rows and
+      // columns of the triangle) into a register min; one atomic per node.
+      const int S = (n * 8 <= gsize) ? 8 : (n * 4 <= gsize) ? 4 : (n * 2 <= gsize) ? 2 : 1;
+      const int per = gsize / S;
+      for (int b0 = 0; b0 < n; b0 += per) {
+        const int i = b0 + tid / S, part = tid & (S - 1);
+        unsigned long long m = ~0ull;
+        int ri = -1;
+        if (i < n) {
+          ri = root[i];
+          const uint32_t rowi = (uint32_t)i * (uint32_t)(i - 1) / 2;
+          for (int k = part; k < n; k += S) {
+            if (root[k] == ri) continue;  // same component, incl. k == i
+            const uint32_t o = (k < i) ? rowi + k : (uint32_t)k * (uint32_t)(k - 1) / 2 + i;
+            const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+            m = key < m ? key : m;
+          }
         }
-        k += gsize;
-        while (k >= j) { k -= j; ++j; }
+        for (int sh2 = 1; sh2 < S; sh2 <<= 1) {
+          const unsigned long long t = shfl_xor_u64(m, sh2);
+          m = t < m ? t : m;
+        }
+        if (i < n && part == 0 && m != ~0ull) atomicMin(&best[ri], m);
       }
     }
     sync();
@@ -497,26 +525,39 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
   }
 }
 
-// One wavefront per net of 3..kSmallDeg pins, four nets per workgroup.
-__global__ __launch_bounds__(256) void k_tree_small(TreeArgs A) {
+// Blocks [0, nb2): one lane per 2-pin net.  Blocks from nb2 on: one
+// wavefront per net of 3..kSmallDeg pins, four nets per workgroup.
+__global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, const int32_t* __restrict__ list2, int n2,
+                                                    int nb2) {
+  TreeArgs A = A0;
+  A.bstamp = *A0.ctr;
+  if ((int)blockIdx.x < nb2) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < n2) tree2_commit(A, list2[idx]);
+    return;
+  }
+  constexpr int kP = kSmallDeg * (kSmallDeg - 1) / 2;
   __shared__ int s_root[4][kSmallDeg];
   __shared__ unsigned long long s_best[4][kSmallDeg];
   __shared__ int s_partner[4][kSmallDeg];
   __shared__ int s_link[4][kSmallDeg];
   __shared__ int2 s_edges[4][kSmallDeg];
   __shared__ float s_ecost[4][kSmallDeg];
+  __shared__ float s_cost[4][kP];
   __shared__ int s_wcnt[4][2];
   __shared__ int s_sh[4][2];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = blockIdx.x * 4 + wv;
+  const int idx = (blockIdx.x - nb2) * 4 + wv;
   if (idx >= A.n_list) return;
   tree_commit<true>(A, A.list[idx], lane, 64, s_root[wv], s_best[wv], s_partner[wv], s_link[wv], s_edges[wv],
-                    s_ecost[wv], s_wcnt[wv], s_sh[wv]);
+                    s_ecost[wv], s_wcnt[wv], s_sh[wv], s_cost[wv], kP);
 }
 
 // One workgroup per net of more than kSmallDeg pins; LDS sized by the largest
 // such net of the batch (cap nodes).
-__global__ __launch_bounds__(256) void k_tree_large(TreeArgs A, int cap) {
+__global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int scap) {
+  TreeArgs A = A0;
+  A.bstamp = *A0.ctr;
   extern __shared__ unsigned long long shl[];
   unsigned long long* best = shl;
   int2* edges = (int2*)(best + cap);
@@ -524,10 +565,11 @@ __global__ __launch_bounds__(256) void k_tree_large(TreeArgs A, int cap) {
   int* partner = root + cap;
   int* link = partner + cap;
   float* ecost = (float*)(link + cap);
-  int* wcnt = (int*)(ecost + cap);  // (cap / 64 + 4) entries
+  __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
+  float* scost = ecost + cap;  // scap floats
   tree_commit<false>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, edges, ecost, wcnt,
-                     s_sh);
+                     s_sh, scost, scap);
 }
 
 }  // namespace grd

@@ -46,13 +46,15 @@ struct BatchMeta {
   int32_t chunk_ofs;      // first entry of the batch in the pair-chunk table
   int32_t cls_ofs[3], cls_n[3];
   int32_t max_large;      // most pins of a large net in the batch
+  int64_t max_large_P;    // most pin pairs of a large net in the batch
 };
 
 struct Sched {
   std::vector<BatchMeta> meta;
   int32_t n_pos = 0;
   int64_t slots = 0, max_P = 0;
-  int32_t* d_net = nullptr;
+  int64_t* d_pinofs = nullptr;  // position -> first pin of the net
+  int32_t* d_deg = nullptr;     // position -> pin count
   int64_t* d_slot = nullptr;
   int64_t* d_pair = nullptr;
   int32_t* d_list = nullptr;
@@ -61,6 +63,8 @@ struct Sched {
   int32_t* d_lpat = nullptr;
   float* d_cost = nullptr;
   size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0;
+  hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
+  bool graph_failed = false;
 };
 
 struct EvPair { hipEvent_t a, b; int kind; };
@@ -78,7 +82,8 @@ struct gr_ctx {
   float lut[kLut];
   int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;
   float *d_pfx = nullptr, *d_lut = nullptr;
-  int32_t bstamp = 0;
+  int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
+  size_t lds_max = 65536;
   int32_t n_nets = 0;
   std::vector<int64_t> h_pin_pfx;
   int64_t* d_pin_pfx = nullptr;
@@ -147,7 +152,7 @@ int refresh_all(gr_ctx* c) {
   {
     KTimer t(c, GR_K_REFRESH);
     hipLaunchKernelGGL(k_refresh, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M, c->d_rem,
-                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, 0, 1);
+                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, 1);
   }
   HIPC(c, hipGetLastError());
   return 0;
@@ -156,7 +161,8 @@ int refresh_all(gr_ctx* c) {
 // Per-position descriptors of a schedule, built on the host and uploaded once.
 int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofst, const int32_t* net_ids) {
   const int32_t npos = batch_ofst[n_batches];
-  std::vector<int64_t> slot(npos), pair(npos);
+  std::vector<int64_t> slot(npos), pair(npos), pinofs(npos);
+  std::vector<int32_t> degv(npos);
   std::vector<int32_t> list(npos), chunk;
   s.meta.assign(n_batches, BatchMeta{});
   int64_t slots = 0, maxP = 0;
@@ -168,6 +174,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     if (m.n < 0) return fail(c, GR_EINVAL, "batch offsets must be non-decreasing");
     int64_t P = 0;
     m.max_large = 0;
+    m.max_large_P = 0;
     for (int i = 0; i < m.n; ++i) {
       const int32_t id = net_ids[m.begin + i];
       if (id < 0 || id >= c->n_nets) return fail(c, GR_EINVAL, "net id out of range");
@@ -175,6 +182,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       if (d > kLargeDeg) return fail(c, GR_ERANGE, "net has more pins than the device tree supports (2048)");
       slot[m.begin + i] = slots;
       pair[m.begin + i] = P;
+      pinofs[m.begin + i] = c->h_pin_pfx[id];
+      degv[m.begin + i] = (int32_t)d;
       slots += d;
       P += d * (d - 1) / 2;
     }
@@ -186,7 +195,10 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
         const int k = d == 2 ? kCls2 : (d <= kSmallDeg ? kClsSmall : kClsLarge);
         if (k != cls) continue;
         list[lo++] = i;
-        if (k == kClsLarge) m.max_large = std::max<int>(m.max_large, (int)d);
+        if (k == kClsLarge) {
+          m.max_large = std::max<int>(m.max_large, (int)d);
+          m.max_large_P = std::max<int64_t>(m.max_large_P, d * (d - 1) / 2);
+        }
       }
       m.cls_n[cls] = lo - m.cls_ofs[cls];
     }
@@ -205,7 +217,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   s.max_P = maxP;
   int rc;
   if ((size_t)npos > s.cap_pos) {
-    if ((rc = dalloc(c, &s.d_net, npos)) || (rc = dalloc(c, &s.d_slot, npos)) || (rc = dalloc(c, &s.d_pair, npos)) ||
+    if ((rc = dalloc(c, &s.d_pinofs, npos)) || (rc = dalloc(c, &s.d_deg, npos)) ||
+        (rc = dalloc(c, &s.d_slot, npos)) || (rc = dalloc(c, &s.d_pair, npos)) ||
         (rc = dalloc(c, &s.d_list, npos)) || (rc = dalloc(c, &s.d_cost, npos)))
       return rc;
     s.cap_pos = npos;
@@ -223,7 +236,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     c->pair_cap = maxP;
   }
   if (npos) {
-    HIPC(c, hipMemcpyAsync(s.d_net, net_ids, sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_pinofs, pinofs.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_deg, degv.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(s.d_slot, slot.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(s.d_pair, pair.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(s.d_list, list.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
@@ -234,8 +248,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   return 0;
 }
 
-typedef void (*LpatFn)(Geo, const float*, const int64_t*, const int32_t*, const int32_t*, const int64_t*,
-                       const int32_t*, int, int64_t, float*, int16_t*);
+typedef void (*LpatFn)(Geo, const float*, const int32_t*, const int64_t*, const int64_t*, const int32_t*, int,
+                       int64_t, float*, int16_t*, int32_t*);
 
 LpatFn lane_kernel(int Z) {
   switch (Z) {
@@ -256,27 +270,26 @@ LpatFn lane_kernel(int Z) {
 int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   const BatchMeta& m = s.meta[b];
   if (m.n == 0) return 0;
-  const int32_t st = ++c->bstamp;
   if (m.P > 0) {
     KTimer t(c, GR_K_LPAT);
     LpatFn fn = lane_kernel(c->g.Z);
     if (fn) {
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + 255) / 256)), dim3(256), 0, c->stream, c->g, c->d_pfx,
-                         c->d_pin_pfx, c->d_pins, s.d_net + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n,
-                         m.P, c->d_pair_cost, c->d_pair_pat);
+                         c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
+                         c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     } else {
       hipLaunchKernelGGL(k_lpat_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
-                         c->d_pin_pfx, c->d_pins, s.d_net + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n,
-                         m.P, c->d_pair_cost, c->d_pair_pat);
+                         c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
+                         c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     }
   }
   TreeArgs A;
   A.g = c->g;
-  A.net = s.d_net + m.begin;
+  A.pinofs = s.d_pinofs + m.begin;
+  A.deg = s.d_deg + m.begin;
   A.slot = s.d_slot + m.begin;
   A.pair = s.d_pair + m.begin;
   A.cost_base = m.begin;
-  A.pin_pfx = c->d_pin_pfx;
   A.pins = c->d_pins;
   A.pair_cost = c->d_pair_cost;
   A.pair_pat = c->d_pair_pat;
@@ -285,33 +298,33 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.out_cost = s.d_cost;
   A.stamp = c->d_stamp;
   A.run_stamp = c->d_run_stamp;
-  A.bstamp = st;
+  A.ctr = c->d_ctr;
+  A.bstamp = 0;
   A.quirk = c->quirk;
   A.M = c->M;
   {
     KTimer t(c, GR_K_TREE);
-    if (m.cls_n[kCls2]) {
-      A.list = s.d_list + m.cls_ofs[kCls2];
-      A.n_list = m.cls_n[kCls2];
-      hipLaunchKernelGGL(k_tree2, dim3((A.n_list + 255) / 256), dim3(256), 0, c->stream, A);
-    }
-    if (m.cls_n[kClsSmall]) {
+    if (m.cls_n[kCls2] || m.cls_n[kClsSmall]) {
+      const int n2 = m.cls_n[kCls2], nb2 = (n2 + 255) / 256;
       A.list = s.d_list + m.cls_ofs[kClsSmall];
       A.n_list = m.cls_n[kClsSmall];
-      hipLaunchKernelGGL(k_tree_small, dim3((A.n_list + 3) / 4), dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL(k_tree_small, dim3(nb2 + (A.n_list + 3) / 4), dim3(256), 0, c->stream, A,
+                         (const int32_t*)(s.d_list + m.cls_ofs[kCls2]), n2, nb2);
     }
     if (m.cls_n[kClsLarge]) {
       A.list = s.d_list + m.cls_ofs[kClsLarge];
       A.n_list = m.cls_n[kClsLarge];
       const int cap = m.max_large;
-      const size_t lds = (size_t)cap * (8 + 8 + 4 + 4 + 4 + 4) + (size_t)(cap / 64 + 4) * 4;
-      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(256), lds, c->stream, A, cap);
+      const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 32) / 4;
+      const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
+      const size_t lds = (size_t)cap * 32 + (size_t)scap * 4;
+      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(1024), lds, c->stream, A, cap, scap);
     }
   }
   {
     KTimer t(c, GR_K_REFRESH);
     hipLaunchKernelGGL(k_refresh, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M, c->d_rem,
-                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, st, 0);
+                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, 0);
   }
   HIPC(c, hipGetLastError());
   c->last_P = m.P;
@@ -319,7 +332,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
 }
 
 void free_sched(Sched& s) {
-  (void)hipFree(s.d_net); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
+  if (s.exec) (void)hipGraphExecDestroy(s.exec);
+  (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
   s = Sched();
 }
@@ -376,14 +390,17 @@ int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_stamp, cs)) ||
-      (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr))) {
+      (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
+      (rc = dalloc(c, &c->d_ctr, 1))) {
     g_err = c->err;
     return bail(rc);
   }
+  c->lds_max = prop.sharedMemPerBlock;
   if (hipFuncSetAttribute((const void*)k_refresh, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)refresh_lds(c)) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_tree_large, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          kLargeDeg * 32 + (kLargeDeg / 64 + 4) * 4) != hipSuccess ||
+                          (int)c->lds_max - 2048) != hipSuccess ||
+      hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemcpy(c->d_cap0, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_rem, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_stamp, 0, sizeof(int32_t) * cs) != hipSuccess ||
@@ -405,7 +422,7 @@ void gr_destroy(gr_ctx* c) {
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_stamp); (void)hipFree(c->d_run_stamp);
-  (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut);
+  (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
   (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat);
   free_sched(c->sched);
@@ -474,8 +491,35 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
   if (first < 0 || count < 0 || (size_t)first + count > c->sched.meta.size())
     return fail(c, GR_EINVAL, "batch range outside the schedule");
   (void)hipSetDevice(c->dev);
+  Sched& s = c->sched;
+  const bool whole = first == 0 && count == (int32_t)s.meta.size() && count > 0;
+  if (whole && !c->timing && !s.graph_failed) {
+    if (!s.exec) {
+      // capture the whole pass once: kernels only (no memset nodes), fixed
+      // arguments; the batch stamp comes from the device counter
+      hipGraph_t graph = nullptr;
+      bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      int rc = 0;
+      if (ok) {
+        for (int32_t b = 0; b < count && !rc; ++b) rc = enqueue_batch(c, s, b);
+        ok = hipStreamEndCapture(c->stream, &graph) == hipSuccess && !rc;
+      }
+      if (ok) ok = hipGraphInstantiate(&s.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!ok) {
+        s.exec = nullptr;
+        s.graph_failed = true;
+        (void)hipGetLastError();
+      }
+    }
+    if (s.exec) {
+      HIPC(c, hipGraphLaunch(s.exec, c->stream));
+      c->last_P = s.meta.back().P;
+      return 0;
+    }
+  }
   for (int32_t b = first; b < first + count; ++b) {
-    const int rc = enqueue_batch(c, c->sched, b);
+    const int rc = enqueue_batch(c, s, b);
     if (rc) return rc;
   }
   return 0;
