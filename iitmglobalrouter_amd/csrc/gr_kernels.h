@@ -525,32 +525,147 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
   }
 }
 
-// Blocks [0, nb2): one lane per 2-pin net.  Blocks from nb2 on: one
-// wavefront per net of 3..kSmallDeg pins, four nets per workgroup.
-__global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, const int32_t* __restrict__ list2, int n2,
-                                                    int nb2) {
+// The same root-pair Boruvka for nets of at most G pins (G in {8,16,32,64}):
+// a group of G lanes per net, node i on lane i, 64/G nets per wavefront.
+// Groups of one wavefront advance in lockstep (wave-wide syncs); a group whose
+// net is complete idles until every group of the wave is.  Phases and their
+// justification are those of tree_commit; the pointer jumping runs a fixed 6
+// doublings (depth < 64).
+struct GroupLds {
+  int root[64];
+  unsigned long long best[64];
+  int partner[64];
+  int link[64];
+  int2 edges[64];
+  float ecost[64];
+  float cost[kSmallDeg * (kSmallDeg - 1) / 2];  // per group: G(G-1)/2 floats
+};
+
+template <int G>
+__device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
+  const int lane = threadIdx.x & 63;
+  const int gb = lane & ~(G - 1), tid = lane - gb;
+  const unsigned long long gbits = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gb);
+  const bool valid = idx < A.n_list;
+  const int pos = valid ? A.list[idx] : 0;
+  const int n = valid ? A.deg[pos] : 0;
+  const int64_t pb = valid ? A.pinofs[pos] : 0;
+  const uint32_t P = (uint32_t)(n * (n - 1) / 2);
+  int* root = L.root + gb;
+  unsigned long long* best = L.best + gb;
+  int* partner = L.partner + gb;
+  int* link = L.link + gb;
+  int2* edges = L.edges + gb;
+  float* ecost = L.ecost + gb;
+  float* cost = L.cost + (gb / G) * (G * (G - 1) / 2);
+  if (valid) {
+    const float* gcost = A.pair_cost + A.pair[pos];
+    for (uint32_t o = tid; o < P; o += G) cost[o] = gcost[o];
+  }
+  if (tid < n) root[tid] = tid;
+  int ne = 0;
+  wave_sync();
+  for (int round = 0; round < G; ++round) {
+    const bool act = ne < n - 1;
+    if (__ballot(act) == 0ull) break;
+    const bool me = act && tid < n;
+    if (me) best[tid] = ~0ull;
+    wave_sync();
+    if (me) {
+      const int ri = root[tid];
+      const uint32_t rowi = (uint32_t)tid * (uint32_t)(tid - 1) / 2;
+      unsigned long long m = ~0ull;
+      for (int k = 0; k < n; ++k) {
+        if (root[k] == ri) continue;
+        const uint32_t o = (k < tid) ? rowi + k : (uint32_t)k * (uint32_t)(k - 1) / 2 + tid;
+        const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+        m = key < m ? key : m;
+      }
+      if (m != ~0ull) atomicMin(&best[ri], m);
+    }
+    wave_sync();
+    int pr = -1;
+    if (me && root[tid] == tid && best[tid] != ~0ull) {
+      int j, k;
+      pair_decode((uint32_t)best[tid], j, k);
+      pr = (root[j] == tid) ? root[k] : root[j];
+    }
+    if (me) partner[tid] = pr;
+    wave_sync();
+    const bool acc = me && pr >= 0 && !(partner[pr] == tid && tid > pr);
+    if (me) link[tid] = acc ? pr : tid;
+    const unsigned long long gm = __ballot(acc) & gbits;
+    if (acc) {
+      const int off = ne + __popcll(gm & ((1ull << lane) - 1ull));
+      edges[off] = make_int2(tid, pr);
+      ecost[off] = cost[(uint32_t)best[tid]];
+    }
+    ne += __popcll(gm);
+    wave_sync();
+#pragma unroll
+    for (int jump = 0; jump < 6; ++jump) {
+      if (me && root[tid] == tid) link[tid] = link[link[tid]];
+      wave_sync();
+    }
+    if (me && root[tid] == tid) partner[tid] = tid;
+    wave_sync();
+    if (me && root[tid] == tid) atomicMax(&partner[link[tid]], tid);
+    wave_sync();
+    const int nr = me ? partner[link[root[tid]]] : 0;
+    wave_sync();
+    if (me) root[tid] = nr;
+    wave_sync();
+  }
+  if (!valid) return;
+  const int64_t sb = A.slot[pos];
+  const int16_t* pat = A.pair_pat + A.pair[pos];
+  const int32_t* npins = A.pins + pb;
+  for (int e = tid; e < n - 1; e += G) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  if (tid == 0) {
+    float nc = 0.0f;
+    for (int e = 0; e < n - 1; ++e) nc += ecost[e];
+    A.out_edges[sb + n - 1] = -1;
+    A.out_lpat[sb + n - 1] = -1;
+    A.out_cost[A.cost_base + pos] = nc;
+  }
+}
+
+// Net classes handled by one k_tree_small launch: class 0 = 2 pins (one lane
+// per net, 256 nets per block); classes 1..4 = up to 8/16/32/64 pins (a group
+// of G lanes per net, 256/G nets per block).  Block ranges per class.
+struct SmallClasses {
+  const int32_t* list[5];
+  int n[5];
+  int block0[6];  // first block of each class; block0[5] = grid size
+};
+
+template <int G>
+__device__ void run_group_class(const TreeArgs& A0, const SmallClasses& C, int cls, GroupLds* lds) {
+  TreeArgs A = A0;
+  A.list = C.list[cls];
+  A.n_list = C.n[cls];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = ((blockIdx.x - C.block0[cls]) * 4 + wv) * (64 / G) + lane / G;
+  tree_group<G>(A, idx, lds[wv]);
+}
+
+__global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C) {
   TreeArgs A = A0;
   A.bstamp = *A0.ctr;
-  if ((int)blockIdx.x < nb2) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx < n2) tree2_commit(A, list2[idx]);
-    return;
+  __shared__ GroupLds lds[4];
+  const int b = blockIdx.x;
+  if (b < C.block0[1]) {
+    const int idx = b * 256 + threadIdx.x;
+    if (idx < C.n[0]) tree2_commit(A, C.list[0][idx]);
+  } else if (b < C.block0[2]) {
+    run_group_class<8>(A, C, 1, lds);
+  } else if (b < C.block0[3]) {
+    run_group_class<16>(A, C, 2, lds);
+  } else if (b < C.block0[4]) {
+    run_group_class<32>(A, C, 3, lds);
+  } else {
+    run_group_class<64>(A, C, 4, lds);
   }
-  constexpr int kP = kSmallDeg * (kSmallDeg - 1) / 2;
-  __shared__ int s_root[4][kSmallDeg];
-  __shared__ unsigned long long s_best[4][kSmallDeg];
-  __shared__ int s_partner[4][kSmallDeg];
-  __shared__ int s_link[4][kSmallDeg];
-  __shared__ int2 s_edges[4][kSmallDeg];
-  __shared__ float s_ecost[4][kSmallDeg];
-  __shared__ float s_cost[4][kP];
-  __shared__ int s_wcnt[4][2];
-  __shared__ int s_sh[4][2];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = (blockIdx.x - nb2) * 4 + wv;
-  if (idx >= A.n_list) return;
-  tree_commit<true>(A, A.list[idx], lane, 64, s_root[wv], s_best[wv], s_partner[wv], s_link[wv], s_edges[wv],
-                    s_ecost[wv], s_wcnt[wv], s_sh[wv], s_cost[wv], kP);
 }
 
 // One workgroup per net of more than kSmallDeg pins; LDS sized by the largest

@@ -37,14 +37,22 @@ void build_lut(int model, float* lut) {
   }
 }
 
-// Net size classes of the tree stage.
-enum { kCls2 = 0, kClsSmall = 1, kClsLarge = 2 };
+// Net size classes of the tree stage: 2 pins; up to 8, 16, 32, 64 pins
+// (one group of that many lanes per net); more (one workgroup per net).
+constexpr int kNumCls = 6;
+constexpr int kClsLarge = 5;
+constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, 1 << 30};
+int net_class(int64_t d) {
+  int k = 0;
+  while (d > kClsMax[k]) ++k;
+  return k;
+}
 
 struct BatchMeta {
   int32_t begin, n;       // schedule positions [begin, begin+n)
   int64_t P;              // pin pairs in the batch
   int32_t chunk_ofs;      // first entry of the batch in the pair-chunk table
-  int32_t cls_ofs[3], cls_n[3];
+  int32_t cls_ofs[kNumCls], cls_n[kNumCls];
   int32_t max_large;      // most pins of a large net in the batch
   int64_t max_large_P;    // most pin pairs of a large net in the batch
 };
@@ -187,12 +195,12 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       slots += d;
       P += d * (d - 1) / 2;
     }
-    for (int cls = 0; cls < 3; ++cls) {
+    for (int cls = 0; cls < kNumCls; ++cls) {
       m.cls_ofs[cls] = lo;
       for (int i = 0; i < m.n; ++i) {
         const int32_t id = net_ids[m.begin + i];
         const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
-        const int k = d == 2 ? kCls2 : (d <= kSmallDeg ? kClsSmall : kClsLarge);
+        const int k = net_class(d);
         if (k != cls) continue;
         list[lo++] = i;
         if (k == kClsLarge) {
@@ -304,13 +312,17 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.M = c->M;
   {
     KTimer t(c, GR_K_TREE);
-    if (m.cls_n[kCls2] || m.cls_n[kClsSmall]) {
-      const int n2 = m.cls_n[kCls2], nb2 = (n2 + 255) / 256;
-      A.list = s.d_list + m.cls_ofs[kClsSmall];
-      A.n_list = m.cls_n[kClsSmall];
-      hipLaunchKernelGGL(k_tree_small, dim3(nb2 + (A.n_list + 3) / 4), dim3(256), 0, c->stream, A,
-                         (const int32_t*)(s.d_list + m.cls_ofs[kCls2]), n2, nb2);
+    SmallClasses C;
+    int blocks = 0;
+    for (int k = 0; k < 5; ++k) {
+      C.list[k] = s.d_list + m.cls_ofs[k];
+      C.n[k] = m.cls_n[k];
+      C.block0[k] = blocks;
+      const int per_block = k == 0 ? 256 : 256 / kClsMax[k];
+      blocks += (C.n[k] + per_block - 1) / per_block;
     }
+    C.block0[5] = blocks;
+    if (blocks) hipLaunchKernelGGL(k_tree_small, dim3(blocks), dim3(256), 0, c->stream, A, C);
     if (m.cls_n[kClsLarge]) {
       A.list = s.d_list + m.cls_ofs[kClsLarge];
       A.n_list = m.cls_n[kClsLarge];
