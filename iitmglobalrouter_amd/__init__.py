@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "load_library", "RouterError", "Router", "gen_grid", "gen_nets", "partition_nets",
+    "load_library", "RouterError", "Router", "Model", "route_cost", "gen_grid", "gen_nets", "partition_nets",
     "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
     "WORKLOADS", "device_count",
 ]
@@ -78,6 +78,20 @@ _SIGS = {
     "gr_gen_grid": (ctypes.c_int, [_I32, _U64, _I32, _I32, _I32, _P]),
     "gr_gen_nets": (ctypes.c_int, [_I32, _U64, _I32, _I32, _I32, _I32, _P, _P, _P, _P]),
     "gr_partition_nets": (ctypes.c_int, [_I32, _I32, _I32, _I32, _P, _P, _I32, _P, _P]),
+    "gr_mlp_create": (ctypes.c_int, [_P, _I32, _U64, _I32, ctypes.POINTER(_P)]),
+    "gr_mlp_destroy": (None, [_P]),
+    "gr_mlp_shape": (ctypes.c_int, [_P, _P]),
+    "gr_mlp_get_params": (ctypes.c_int, [_P, _P]),
+    "gr_mlp_set_params": (ctypes.c_int, [_P, _P]),
+    "gr_mlp_collect": (ctypes.c_int, [_P, _P, _I32]),
+    "gr_mlp_samples": (_I64, [_P]),
+    "gr_mlp_clear_samples": (ctypes.c_int, [_P]),
+    "gr_mlp_train": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _U64, ctypes.POINTER(ctypes.c_float)]),
+    "gr_mlp_features": (ctypes.c_int, [_P, _P, _P, _P, _I32, _P]),
+    "gr_mlp_forward": (ctypes.c_int, [_P, _P, _I32, _P]),
+    "gr_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _I32, _P, ctypes.POINTER(ctypes.c_float)]),
+    "gr_set_proposal": (ctypes.c_int, [_P, _P, _I32, ctypes.c_float]),
+    "gr_proposal_stats": (ctypes.c_int, [_P, _P, _P]),
 }
 
 
@@ -144,6 +158,17 @@ def partition_nets(X, Y, Z, pin_pfx, pin_enc, num_xdim_parts=4):
                                       _ptr(np.ascontiguousarray(pin_enc, dtype=np.int32)), num_xdim_parts,
                                       _ptr(ofst), _ptr(ids)))
     return ofst[: nb + 1].copy(), ids[:n].copy()
+
+
+def route_cost(cap, demand):
+    """Route quality of a finished pass: wirelength = committed cell-uses (sum
+    of demand), overflow = sum over cells of max(0, demand - capacity), and the
+    total route cost wirelength + overflow."""
+    cap = np.asarray(cap, np.int64)
+    d = np.asarray(demand, np.int64)
+    wl = int(d.sum())
+    of = int(np.maximum(d - cap, 0).sum())
+    return {"wirelength": wl, "overflow": of, "total": wl + of}
 
 
 class Router:
@@ -241,6 +266,16 @@ class Router:
         self._c(self.lib.gr_get_lut(self.h, _ptr(out)))
         return out
 
+    def set_proposal(self, model, k=8, threshold=0.6):
+        """Cost only the model's top-k patterns (None: exhaustive costing)."""
+        self._c(self.lib.gr_set_proposal(self.h, model.h if model is not None else None, k, threshold))
+
+    def proposal_stats(self):
+        p = ctypes.c_int64()
+        f = ctypes.c_int64()
+        self._c(self.lib.gr_proposal_stats(self.h, ctypes.byref(p), ctypes.byref(f)))
+        return p.value, f.value
+
     def enable_timing(self, on=True):
         self._c(self.lib.gr_enable_timing(self.h, 1 if on else 0))
 
@@ -249,3 +284,74 @@ class Router:
         ms = np.zeros(3, dtype=np.float64)
         self._c(self.lib.gr_kernel_stats(self.h, _ptr(n), _ptr(ms)))
         return n, ms
+
+
+class Model:
+    """Learned proposal: MLP (4 + 25 Z) -> hidden -> hidden -> (Z-1) ceil(Z/2)
+    scoring L-pattern candidates; bf16 MFMA kernels, trained on the GPU."""
+
+    def __init__(self, router, hidden=256, seed=0, max_samples=1 << 20):
+        self.router = router
+        self.lib = router.lib
+        h = ctypes.c_void_p()
+        _check(self.lib.gr_mlp_create(router.h, hidden, seed, max_samples, ctypes.byref(h)), router.h)
+        self.h = h
+        s = np.zeros(6, np.int32)
+        self.lib.gr_mlp_shape(self.h, _ptr(s))
+        self.n_in, self.k_in, self.hidden, self.n_out, self.n_out_pad, self.n_params = (int(v) for v in s)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, rc):
+        return _check(rc, self.router.h)
+
+    def params(self):
+        out = np.empty(self.n_params, np.float32)
+        self._c(self.lib.gr_mlp_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._c(self.lib.gr_mlp_set_params(self.h, _ptr(p)))
+
+    def collect(self, per_batch):
+        self._c(self.lib.gr_mlp_collect(self.router.h, self.h if per_batch else None, per_batch))
+
+    def samples(self):
+        return self.lib.gr_mlp_samples(self.h)
+
+    def clear_samples(self):
+        self._c(self.lib.gr_mlp_clear_samples(self.h))
+
+    def train(self, epochs=1, batch=1024, lr=0.05, seed=0):
+        loss = ctypes.c_float()
+        self._c(self.lib.gr_mlp_train(self.h, epochs, batch, lr, seed, ctypes.byref(loss)))
+        return loss.value
+
+    def features(self, src, dst):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        out = np.empty((len(src), self.k_in), np.float32)
+        self._c(self.lib.gr_mlp_features(self.router.h, self.h, _ptr(src), _ptr(dst), len(src), _ptr(out)))
+        return out
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty((x.shape[0], self.n_out_pad), np.float32)
+        self._c(self.lib.gr_mlp_forward(self.h, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def gradient(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        self._c(self.lib.gr_mlp_gradient(self.h, _ptr(x), _ptr(y), x.shape[0], _ptr(g), ctypes.byref(loss)))
+        return g, loss.value

@@ -104,6 +104,37 @@ __device__ __forceinline__ void pair_decode(uint32_t o, int& j, int& k) {
   k = (int)(o - (uint32_t)((uint64_t)jj * (jj - 1) / 2));
 }
 
+// Single-cell via term at (x, y) on layer i: Q(pos) - Q(pos - 1), bit_util.hpp:108-117.
+__device__ __forceinline__ float dterm(const float* __restrict__ pfx, const Geo& g, int i, int x, int y) {
+  return (i & 1) ? Q(pfx, g, i, x, y) - Q(pfx, g, i, x, y - 1) : Q(pfx, g, i, y, x) - Q(pfx, g, i, y, x - 1);
+}
+// Inclusive wire u..v on a run: Q(hi) - Q(lo - 1), bit_util.hpp:91-107.
+__device__ __forceinline__ float wterm(const float* __restrict__ pfx, const Geo& g, int z, int line, int u, int v) {
+  const int lo = u < v ? u : v, hi = u < v ? v : u;
+  return Q(pfx, g, z, line, hi) - Q(pfx, g, z, line, lo - 1);
+}
+
+// Pair w of the batch -> batch position of the net that owns it.  chunk_net[c]
+// is the owner of pair 64*c; a net has >= 1 pair, so the owner of any pair of
+// chunk c is among the 64 positions starting there.  The wavefront of chunk c
+// loads that window of pair offsets once (coalesced) and each lane binary
+// searches it through lane shuffles.  Every lane of the wave must call this.
+__device__ __forceinline__ int pair_owner_wave(const int64_t* __restrict__ bpair,
+                                               const int32_t* __restrict__ chunk_net, int n, int64_t w) {
+  const int lane = threadIdx.x & 63;
+  const int base = chunk_net[w >> 6];
+  const int64_t v = (base + lane < n) ? bpair[base + lane] : INT64_MAX;
+  int lo = 0, hi = 64;  // v[lo] <= w < v[hi]
+#pragma unroll
+  for (int it = 0; it < 6; ++it) {
+    const int mid = (lo + hi) >> 1;
+    const int64_t vm = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(v >> 32), mid, 64) << 32) |
+                                 (uint32_t)__shfl((int)v, mid, 64));
+    if (vm <= w) lo = mid; else hi = mid;
+  }
+  return base + lo;
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
   lo = __shfl_xor(lo, m, 64);

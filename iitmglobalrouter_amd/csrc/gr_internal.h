@@ -1,0 +1,125 @@
+// Internal state shared by the routing (gr_route.hip) and learned-proposal
+// (gr_mlp.hip) translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/iitmgr.h"
+#include "gr_device.h"
+
+namespace grh {
+
+using grd::Geo;
+using grd::kLut;
+
+// Net size classes of the tree stage: 2 pins; up to 8, 16, 32, 64 pins
+// (one group of that many lanes per net); more (one workgroup per net).
+constexpr int kNumCls = 6;
+constexpr int kClsLarge = 5;
+constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, 1 << 30};
+inline int net_class(int64_t d) {
+  int k = 0;
+  while (d > kClsMax[k]) ++k;
+  return k;
+}
+
+struct BatchMeta {
+  int32_t begin, n;       // schedule positions [begin, begin+n)
+  int64_t P;              // pin pairs in the batch
+  int32_t chunk_ofs;      // first entry of the batch in the pair-chunk table
+  int32_t cls_ofs[kNumCls], cls_n[kNumCls];
+  int32_t max_large;      // most pins of a large net in the batch
+  int64_t max_large_P;    // most pin pairs of a large net in the batch
+};
+
+struct Sched {
+  std::vector<BatchMeta> meta;
+  int32_t n_pos = 0;
+  int64_t slots = 0, max_P = 0;
+  int64_t* d_pinofs = nullptr;  // position -> first pin of the net
+  int32_t* d_deg = nullptr;     // position -> pin count
+  int64_t* d_slot = nullptr;
+  int64_t* d_pair = nullptr;
+  int32_t* d_list = nullptr;
+  int32_t* d_chunk = nullptr;
+  int64_t* d_edges = nullptr;
+  int32_t* d_lpat = nullptr;
+  float* d_cost = nullptr;
+  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0;
+  hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
+  bool graph_failed = false;
+};
+
+struct EvPair { hipEvent_t a, b; int kind; };
+
+}  // namespace grh
+
+struct gr_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  grd::Geo g{};
+  int model = 0, quirk = 1, M = 0;
+  int64_t cs = 0;
+  float lut[grd::kLut];
+  int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;
+  float *d_pfx = nullptr, *d_lut = nullptr;
+  int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
+  size_t lds_max = 65536;
+  int32_t n_nets = 0;
+  std::vector<int64_t> h_pin_pfx;
+  int64_t* d_pin_pfx = nullptr;
+  int32_t* d_pins = nullptr;
+  grh::Sched sched, single;
+  float* d_pair_cost = nullptr;
+  int16_t* d_pair_pat = nullptr;
+  int64_t pair_cap = 0, last_P = 0;
+  // learned proposal: when set, costing evaluates only the model's top-k patterns
+  gr_mlp* prop = nullptr;
+  int prop_k = 8;
+  float prop_thr = 0.6f;
+  int64_t* d_prop_stats = nullptr;  // [pairs costed, pairs that fell back to full enumeration]
+  // training-data capture after each batch's costing
+  gr_mlp* collect = nullptr;
+  int collect_per_batch = 0;
+  bool timing = false;
+  std::vector<grh::EvPair> evs;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::string err;
+};
+
+
+namespace grh {
+
+extern std::string g_err;
+
+inline int fail(gr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg; else g_err = msg;
+  return code;
+}
+
+#define HIPC(c, x)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      return grh::fail((c), GR_ENODEV, std::string(#x " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(gr_ctx* c, T** p, size_t n) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) { *p = nullptr; return fail(c, GR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+  return 0;
+}
+
+// Learned-proposal hooks (gr_mlp.hip): costing of one batch by the model's
+// top-k patterns, and capture of (features, chosen pattern) training samples.
+int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& m);
+int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& m);
+
+}  // namespace grh

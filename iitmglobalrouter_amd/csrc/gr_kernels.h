@@ -88,37 +88,6 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // L-pattern costing
 // ---------------------------------------------------------------------------
 
-// Single-cell via term at (x, y) on layer i: Q(pos) - Q(pos - 1), bit_util.hpp:108-117.
-__device__ __forceinline__ float dterm(const float* __restrict__ pfx, const Geo& g, int i, int x, int y) {
-  return (i & 1) ? Q(pfx, g, i, x, y) - Q(pfx, g, i, x, y - 1) : Q(pfx, g, i, y, x) - Q(pfx, g, i, y, x - 1);
-}
-// Inclusive wire u..v on a run: Q(hi) - Q(lo - 1), bit_util.hpp:91-107.
-__device__ __forceinline__ float wterm(const float* __restrict__ pfx, const Geo& g, int z, int line, int u, int v) {
-  const int lo = u < v ? u : v, hi = u < v ? v : u;
-  return Q(pfx, g, z, line, hi) - Q(pfx, g, z, line, lo - 1);
-}
-
-// Pair w of the batch -> batch position of the net that owns it.  chunk_net[c]
-// is the owner of pair 64*c; a net has >= 1 pair, so the owner of any pair of
-// chunk c is among the 64 positions starting there.  The wavefront of chunk c
-// loads that window of pair offsets once (coalesced) and each lane binary
-// searches it through lane shuffles.  Every lane of the wave must call this.
-__device__ __forceinline__ int pair_owner_wave(const int64_t* __restrict__ bpair,
-                                               const int32_t* __restrict__ chunk_net, int n, int64_t w) {
-  const int lane = threadIdx.x & 63;
-  const int base = chunk_net[w >> 6];
-  const int64_t v = (base + lane < n) ? bpair[base + lane] : INT64_MAX;
-  int lo = 0, hi = 64;  // v[lo] <= w < v[hi]
-#pragma unroll
-  for (int it = 0; it < 6; ++it) {
-    const int mid = (lo + hi) >> 1;
-    const int64_t vm = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(v >> 32), mid, 64) << 32) |
-                                 (uint32_t)__shfl((int)v, mid, 64));
-    if (vm <= w) lo = mid; else hi = mid;
-  }
-  return base + lo;
-}
-
 // One lane per pin pair.  All via and wire terms of the pair are loaded once
 // into registers; patterns are then evaluated in ascending p with exactly the
 // oracle's fp32 operation sequence (bit_util.hpp:131-179):
@@ -545,7 +514,7 @@ template <int G>
 __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
   const int lane = threadIdx.x & 63;
   const int gb = lane & ~(G - 1), tid = lane - gb;
-  const unsigned long long gbits = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gb);
+  const unsigned long long gbits = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << gb);
   const bool valid = idx < A.n_list;
   const int pos = valid ? A.list[idx] : 0;
   const int n = valid ? A.deg[pos] : 0;

@@ -1,0 +1,729 @@
+// Learned proposal for L-pattern costing (no counterpart in the reference):
+// a 3-layer MLP scores the candidate patterns of each pin pair so that only
+// its top-k are costed exactly.  Hand-written gfx950 kernels: per-pair feature
+// gather, a bf16 MFMA GEMM (v_mfma_f32_16x16x32_bf16, fp32 accumulation) used
+// for the forward pass and every backward product, softmax cross-entropy,
+// momentum SGD on fp32 master weights, and the top-k costing kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "gr_internal.h"
+
+using namespace grd;
+using namespace grh;
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kMaxTopK = 8;
+
+// bf16 <-> fp32 by bit manipulation (round to nearest even); identical on host and device.
+__host__ __device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u;
+#ifdef __HIP_DEVICE_COMPILE__
+  u = __float_as_uint(x);
+#else
+  std::memcpy(&u, &x, 4);
+#endif
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+#ifdef __HIP_DEVICE_COMPILE__
+  return __uint_as_float(u);
+#else
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Features
+// ---------------------------------------------------------------------------
+
+// Row of k_in bf16 features for the pair src -> dst:
+//   [0] dx/64, [1] dy/64, [2] z_src, [3] z_dst,
+//   [4 + l*25 + u*5 + v] log cost of cell (l, x1 + round(u*dx/4), y1 + round(v*dy/4)),
+//   zero padding to k_in.
+__device__ void feat_row(const Geo& g, const int32_t* __restrict__ rem, const float* __restrict__ loglut,
+                         uint32_t src, uint32_t dst, uint16_t* __restrict__ row, int k_in) {
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, src, z1, x1, y1);
+  dec(g, dst, z2, x2, y2);
+  const int dx = x2 - x1, dy = y2 - y1;
+  row[0] = f2bf((float)dx * (1.0f / 64.0f));
+  row[1] = f2bf((float)dy * (1.0f / 64.0f));
+  row[2] = f2bf((float)z1);
+  row[3] = f2bf((float)z2);
+  int xs[5], ys[5];
+#pragma unroll
+  for (int u = 0; u < 5; ++u) {
+    xs[u] = x1 + (dx * u + (dx >= 0 ? 2 : -2)) / 4;
+    ys[u] = y1 + (dy * u + (dy >= 0 ? 2 : -2)) / 4;
+  }
+  int f = 4;
+  for (int l = 0; l < g.Z; ++l)
+    for (int u = 0; u < 5; ++u)
+      for (int v = 0; v < 5; ++v) {
+        int r = rem[enc(g, l, xs[u], ys[v])];
+        r = r < kRMin ? kRMin : (r > kRMin + kLut - 1 ? kRMin + kLut - 1 : r);
+        row[f++] = f2bf(loglut[r - kRMin]);
+      }
+  for (; f < k_in; ++f) row[f] = 0;
+}
+
+// Features of every pair of a batch (costing) or of a strided sample of them
+// (training capture: sample i is pair i*stride; its label is that pair's
+// exhaustive pattern minus Z2).
+__global__ __launch_bounds__(256) void k_feat_batch(Geo g, const int32_t* __restrict__ rem,
+                                                    const float* __restrict__ loglut,
+                                                    const int32_t* __restrict__ pins,
+                                                    const int64_t* __restrict__ pinofs,
+                                                    const int64_t* __restrict__ bpair, int n, int64_t count,
+                                                    int64_t stride, uint16_t* __restrict__ X, int k_in,
+                                                    const int16_t* __restrict__ pat, int32_t* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  const int64_t w = i * stride;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bpair[mid] <= w) lo = mid; else hi = mid;
+  }
+  int j, k;
+  pair_decode((uint32_t)(w - bpair[lo]), j, k);
+  const int64_t pb = pinofs[lo];
+  feat_row(g, rem, loglut, (uint32_t)pins[pb + j], (uint32_t)pins[pb + k], X + i * k_in, k_in);
+  if (y) y[i] = (int32_t)pat[w] - g.Z2;
+}
+
+__global__ __launch_bounds__(256) void k_feat_pairs(Geo g, const int32_t* __restrict__ rem,
+                                                    const float* __restrict__ loglut,
+                                                    const int32_t* __restrict__ src,
+                                                    const int32_t* __restrict__ dst, int n,
+                                                    uint16_t* __restrict__ X, int k_in) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  feat_row(g, rem, loglut, (uint32_t)src[i], (uint32_t)dst[i], X + (int64_t)i * k_in, k_in);
+}
+
+// ---------------------------------------------------------------------------
+// bf16 MFMA GEMM
+// ---------------------------------------------------------------------------
+
+// C[M x N] = epilogue(A[M x K] * B[K x N]); A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn]; K % 32 == 0.  Epilogue: + bias[n], ReLU,
+// zero where mask(m,n) <= 0, store bf16 or fp32.  A 256-thread block
+// computes a 64x64 tile as 2x2 wavefronts of 32x32, each 2x2 MFMA
+// 16x16x32 fragments: lane l holds A[l%16][8*(l/16) .. +8), B[8*(l/16) ..
+// +8)[l%16] and D[4*(l/16) + i][l%16], i < 4.
+template <bool RELU, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int64_t sam,
+                                              int64_t sak, const uint16_t* __restrict__ B, int64_t sbk, int64_t sbn,
+                                              const float* __restrict__ bias, const uint16_t* __restrict__ mask,
+                                              int64_t ldm, uint16_t* __restrict__ Cb, float* __restrict__ Cf,
+                                              int64_t ldc) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  v4f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+  const int lr = t >> 2, lk = (t & 3) * 8;
+  const int gm = m0 + lr, gn = n0 + lr;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    if (gm < M && sak == 1) {
+      *(uint4*)&As[lr][lk] = *(const uint4*)(A + (int64_t)gm * sam + k0 + lk);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[lr][lk + i] = gm < M ? A[(int64_t)gm * sam + (int64_t)(k0 + lk + i) * sak] : 0;
+    }
+    if (gn < N && sbk == 1) {
+      *(uint4*)&Bs[lr][lk] = *(const uint4*)(B + (int64_t)gn * sbn + k0 + lk);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[lr][lk + i] = gn < N ? B[(int64_t)(k0 + lk + i) * sbk + (int64_t)gn * sbn] : 0;
+    }
+    __syncthreads();
+    v8bf fa[2], fb[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      fa[f] = __builtin_bit_cast(v8bf, *(const uint4*)&As[wm + f * 16 + (lane & 15)][8 * (lane >> 4)]);
+      fb[f] = __builtin_bit_cast(v8bf, *(const uint4*)&Bs[wn + f * 16 + (lane & 15)][8 * (lane >> 4)]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = m0 + wm + a * 16 + 4 * (lane >> 4) + i;
+        const int col = n0 + wn + b * 16 + (lane & 15);
+        if (row >= M || col >= N) continue;
+        float v = acc[a][b][i];
+        if (bias) v += bias[col];
+        if (RELU) v = v > 0.f ? v : 0.f;
+        if (mask && !(bf2f(mask[(int64_t)row * ldm + col]) > 0.f)) v = 0.f;
+        if (OUT_F32) Cf[(int64_t)row * ldc + col] = v;
+        else Cb[(int64_t)row * ldc + col] = f2bf(v);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Loss, bias gradients, update
+// ---------------------------------------------------------------------------
+
+// Softmax cross-entropy over the first n_out logits of each row; labels < 0
+// mark padding rows.  dL = (softmax - onehot) / n_real as bf16 (padding
+// columns 0).  loss_sum += -log p[y] (sum over rows).
+__global__ __launch_bounds__(256) void k_softmax_ce(int rows, int n_out, int ld, const float* __restrict__ L,
+                                                    const int32_t* __restrict__ y, float inv_n,
+                                                    uint16_t* __restrict__ dL, float* __restrict__ loss_sum) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= rows) return;
+  const float* l = L + (int64_t)r * ld;
+  uint16_t* d = dL + (int64_t)r * ld;
+  const int lab = y[r];
+  if (lab < 0) {
+    for (int j = 0; j < ld; ++j) d[j] = 0;
+    return;
+  }
+  float mx = l[0];
+  for (int j = 1; j < n_out; ++j) mx = fmaxf(mx, l[j]);
+  float s = 0.f;
+  for (int j = 0; j < n_out; ++j) s += expf(l[j] - mx);
+  const float inv = 1.f / s;
+  for (int j = 0; j < n_out; ++j) {
+    const float p = expf(l[j] - mx) * inv;
+    d[j] = f2bf((p - (j == lab ? 1.f : 0.f)) * inv_n);
+  }
+  for (int j = n_out; j < ld; ++j) d[j] = 0;
+  atomicAdd(loss_sum, -(l[lab] - mx - logf(s)));
+}
+
+// db[j] = sum over rows of dZ[row][j] (bf16 values, fp32 sum in row order).
+__global__ __launch_bounds__(256) void k_colsum(int rows, int cols, const uint16_t* __restrict__ dZ, int64_t ld,
+                                                float* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += bf2f(dZ[(int64_t)r * ld + j]);
+  out[j] = s;
+}
+
+// Momentum SGD on the fp32 master parameters.
+__global__ __launch_bounds__(256) void k_sgd(int n, float* __restrict__ w, float* __restrict__ mom,
+                                             const float* __restrict__ g, float lr) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float m = 0.9f * mom[i] + g[i];
+  mom[i] = m;
+  w[i] -= lr * m;
+}
+
+// bf16 copies of a [K x N] fp32 weight: row-major Wb and transposed Wt.
+__global__ __launch_bounds__(256) void k_to_bf16(int K, int N, const float* __restrict__ W, uint16_t* __restrict__ Wb,
+                                                 uint16_t* __restrict__ Wt) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= K * N) return;
+  const int k = i / N, n = i - k * N;
+  const uint16_t h = f2bf(W[i]);
+  Wb[i] = h;
+  Wt[(int64_t)n * K + k] = h;
+}
+
+__global__ __launch_bounds__(256) void k_gather_rows(int rows, int k_in, const int32_t* __restrict__ perm,
+                                                     const uint16_t* __restrict__ X, const int32_t* __restrict__ y,
+                                                     uint16_t* __restrict__ Xo, int32_t* __restrict__ yo) {
+  const int r = blockIdx.x;
+  if (r >= rows) return;
+  const int64_t src = perm[r];
+  for (int f = threadIdx.x; f < k_in; f += 256) Xo[(int64_t)r * k_in + f] = X[src * k_in + f];
+  if (threadIdx.x == 0) yo[r] = y[src];
+}
+
+// ---------------------------------------------------------------------------
+// Top-k costing
+// ---------------------------------------------------------------------------
+
+// One lane per pair: the k largest logits (ties: lower index first) name the
+// candidate patterns p = q + Z2; each is costed exactly and the lexicographic
+// minimum of (cost, p) wins.  When the top-1 softmax probability is below
+// thr, or no candidate is a valid route, every pattern is costed.
+__global__ __launch_bounds__(256) void k_lpat_topk(Geo g, const float* __restrict__ pfx,
+                                                   const int32_t* __restrict__ pins,
+                                                   const int64_t* __restrict__ pinofs,
+                                                   const int64_t* __restrict__ bpair,
+                                                   const int32_t* __restrict__ chunk_net, int n, int64_t P,
+                                                   const float* __restrict__ logits, int ld, int n_out, int topk,
+                                                   float thr, float* __restrict__ out_cost,
+                                                   int16_t* __restrict__ out_pat, int32_t* __restrict__ ctr,
+                                                   unsigned long long* __restrict__ stats) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp
+  const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t w = w0 < P ? w0 : P - 1;
+  const int pos = pair_owner_wave(bpair, chunk_net, n, w);
+  bool fb = false;
+  if (w0 < P) {
+    int j, k;
+    pair_decode((uint32_t)(w - bpair[pos]), j, k);
+    const int64_t pb = pinofs[pos];
+    int z1, x1, y1, z2, x2, y2;
+    dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
+    dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
+    const float* l = logits + w * ld;
+    float tv[kMaxTopK];
+    int ti[kMaxTopK];
+#pragma unroll
+    for (int t = 0; t < kMaxTopK; ++t) { tv[t] = -INFINITY; ti[t] = -1; }
+    float mx = -INFINITY;
+    for (int q = 0; q < n_out; ++q) {
+      float v = l[q];
+      mx = fmaxf(mx, v);
+      int vi = q;
+#pragma unroll
+      for (int t = 0; t < kMaxTopK; ++t) {  // insertion keeps (value desc, index asc)
+        if (t < topk && (v > tv[t] || ti[t] < 0)) {
+          const float sv = tv[t];
+          const int si = ti[t];
+          tv[t] = v; ti[t] = vi;
+          v = sv; vi = si;
+        }
+      }
+    }
+    float s = 0.f;
+    for (int q = 0; q < n_out; ++q) s += expf(l[q] - mx);
+    uint64_t best = ~0ull;
+    fb = (1.0f / s) < thr;
+    if (!fb) {
+#pragma unroll
+      for (int t = 0; t < kMaxTopK; ++t) {
+        if (t >= topk || ti[t] < 0) continue;
+        const int p = ti[t] + g.Z2;
+        float c;
+        if (pattern_cost(pfx, g, z1, x1, y1, z2, x2, y2, p, c)) {
+          const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)p;
+          best = key < best ? key : best;
+        }
+      }
+      fb = best == ~0ull;
+    }
+    if (fb) {
+      for (int p = g.Z2; p < g.Z * g.Z2; ++p) {
+        float c;
+        if (pattern_cost(pfx, g, z1, x1, y1, z2, x2, y2, p, c)) {
+          const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)p;
+          best = key < best ? key : best;
+        }
+      }
+    }
+    out_cost[w] = key_cost((uint32_t)(best >> 32));
+    out_pat[w] = (int16_t)(uint32_t)best;
+  }
+  const unsigned long long act = __ballot(w0 < P), fbm = __ballot(fb && w0 < P);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&stats[0], (unsigned long long)__popcll(act));
+    atomicAdd(&stats[1], (unsigned long long)__popcll(fbm));
+  }
+}
+
+int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+struct gr_mlp {
+  gr_ctx* ctx = nullptr;
+  int n_in = 0, k_in = 0, H = 0, n_out = 0, n_op = 0;  // n_op: n_out padded to 64
+  int n_params = 0;
+  size_t oW1, ob1, oW2, ob2, oW3, ob3;
+  float *d_master = nullptr, *d_mom = nullptr, *d_grad = nullptr, *d_loglut = nullptr, *d_loss = nullptr;
+  uint16_t *W1b = nullptr, *W1t = nullptr, *W2b = nullptr, *W2t = nullptr, *W3b = nullptr, *W3t = nullptr;
+  int64_t cap_samples = 0, n_samples = 0;
+  uint16_t* d_X = nullptr;
+  int32_t* d_y = nullptr;
+  int64_t act_cap = 0;  // rows of the inference / training activation buffers
+  uint16_t *a_X = nullptr, *a_H1 = nullptr, *a_H2 = nullptr, *a_dL = nullptr, *a_dZ2 = nullptr, *a_dZ1 = nullptr;
+  float* a_L = nullptr;
+  int32_t *a_y = nullptr, *d_perm = nullptr;
+};
+
+namespace {
+
+int mlp_reserve(gr_mlp* m, int64_t rows) {
+  if (rows <= m->act_cap) return 0;
+  gr_ctx* c = m->ctx;
+  int rc;
+  if ((rc = dalloc(c, &m->a_X, rows * m->k_in)) || (rc = dalloc(c, &m->a_H1, rows * m->H)) ||
+      (rc = dalloc(c, &m->a_H2, rows * m->H)) || (rc = dalloc(c, &m->a_L, rows * m->n_op)) ||
+      (rc = dalloc(c, &m->a_dL, rows * m->n_op)) || (rc = dalloc(c, &m->a_dZ2, rows * m->H)) ||
+      (rc = dalloc(c, &m->a_dZ1, rows * m->H)) || (rc = dalloc(c, &m->a_y, rows)) || (rc = dalloc(c, &m->d_perm, rows)))
+    return rc;
+  m->act_cap = rows;
+  return 0;
+}
+
+template <bool RELU, bool OUT_F32>
+void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+          int64_t sbk, int64_t sbn, const float* bias, const uint16_t* mask, int64_t ldm, uint16_t* Cb, float* Cf,
+          int64_t ldc) {
+  hipLaunchKernelGGL((k_gemm<RELU, OUT_F32>), dim3(blocks_for(M, 64), blocks_for(N, 64)), dim3(256), 0, st, M, N, K,
+                     A, sam, sak, B, sbk, sbn, bias, mask, ldm, Cb, Cf, ldc);
+}
+
+void refresh_bf16(gr_mlp* m, hipStream_t st) {
+  const float* w = m->d_master;
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->k_in * m->H, 256)), dim3(256), 0, st, m->k_in, m->H,
+                     w + m->oW1, m->W1b, m->W1t);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->H * m->H, 256)), dim3(256), 0, st, m->H, m->H,
+                     w + m->oW2, m->W2b, m->W2t);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->H * m->n_op, 256)), dim3(256), 0, st, m->H, m->n_op,
+                     w + m->oW3, m->W3b, m->W3t);
+}
+
+// Forward pass of rows x a_X -> a_H1, a_H2, a_L (fp32 logits).
+void forward(gr_mlp* m, hipStream_t st, int64_t rows, const uint16_t* X) {
+  const float* w = m->d_master;
+  gemm<true, false>(st, (int)rows, m->H, m->k_in, X, m->k_in, 1, m->W1t, 1, m->k_in, w + m->ob1, nullptr, 0, m->a_H1,
+                    nullptr, m->H);
+  gemm<true, false>(st, (int)rows, m->H, m->H, m->a_H1, m->H, 1, m->W2t, 1, m->H, w + m->ob2, nullptr, 0, m->a_H2,
+                    nullptr, m->H);
+  gemm<false, true>(st, (int)rows, m->n_op, m->H, m->a_H2, m->H, 1, m->W3t, 1, m->H, w + m->ob3, nullptr, 0, nullptr,
+                    m->a_L, m->n_op);
+}
+
+// Loss and gradient of rows (multiple of 32) in a_X / a_y (padding labels -1);
+// gradient into d_grad, loss sum into d_loss.
+void backward(gr_mlp* m, hipStream_t st, int rows, int n_real) {
+  float* gr = m->d_grad;
+  hipLaunchKernelGGL(k_softmax_ce, dim3(blocks_for(rows, 256)), dim3(256), 0, st, rows, m->n_out, m->n_op, m->a_L,
+                     m->a_y, 1.0f / (float)n_real, m->a_dL, m->d_loss);
+  // layer 3: dW3 = H2^T dL, db3 = colsum dL, dZ2 = (dL W3^T) * [H2 > 0]
+  gemm<false, true>(st, m->H, m->n_op, rows, m->a_H2, 1, m->H, m->a_dL, m->n_op, 1, nullptr, nullptr, 0, nullptr,
+                    gr + m->oW3, m->n_op);
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->n_op, 256)), dim3(256), 0, st, rows, m->n_op, m->a_dL,
+                     (int64_t)m->n_op, gr + m->ob3);
+  gemm<false, false>(st, rows, m->H, m->n_op, m->a_dL, m->n_op, 1, m->W3b, 1, m->n_op, nullptr, m->a_H2, m->H,
+                     m->a_dZ2, nullptr, m->H);
+  // layer 2
+  gemm<false, true>(st, m->H, m->H, rows, m->a_H1, 1, m->H, m->a_dZ2, m->H, 1, nullptr, nullptr, 0, nullptr,
+                    gr + m->oW2, m->H);
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->H, 256)), dim3(256), 0, st, rows, m->H, m->a_dZ2, (int64_t)m->H,
+                     gr + m->ob2);
+  gemm<false, false>(st, rows, m->H, m->H, m->a_dZ2, m->H, 1, m->W2b, 1, m->H, nullptr, m->a_H1, m->H, m->a_dZ1,
+                     nullptr, m->H);
+  // layer 1
+  gemm<false, true>(st, m->k_in, m->H, rows, m->a_X, 1, m->k_in, m->a_dZ1, m->H, 1, nullptr, nullptr, 0, nullptr,
+                    gr + m->oW1, m->H);
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->H, 256)), dim3(256), 0, st, rows, m->H, m->a_dZ1, (int64_t)m->H,
+                     gr + m->ob1);
+}
+
+}  // namespace
+
+namespace grh {
+
+// Exhaustive costing already ran; capture a strided sample of the batch's
+// pairs with their chosen patterns as labels.
+int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& bm) {
+  gr_mlp* m = c->collect;
+  const int64_t room = m->cap_samples - m->n_samples;
+  if (room <= 0 || bm.P <= 0) return 0;
+  int64_t want = std::min<int64_t>({(int64_t)c->collect_per_batch, bm.P, room});
+  const int64_t stride = std::max<int64_t>(1, bm.P / want);
+  want = std::min<int64_t>(want, (bm.P + stride - 1) / stride);
+  hipLaunchKernelGGL(k_feat_batch, dim3(blocks_for(want, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
+                     c->d_pins, s.d_pinofs + bm.begin, s.d_pair + bm.begin, bm.n, want, stride,
+                     m->d_X + m->n_samples * m->k_in, m->k_in, (const int16_t*)c->d_pair_pat, m->d_y + m->n_samples);
+  m->n_samples += want;
+  HIPC(c, hipGetLastError());
+  return 0;
+}
+
+// Learned costing of one batch: features -> MLP -> top-k exact costing.
+int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
+  gr_mlp* m = c->prop;
+  int rc;
+  if ((rc = mlp_reserve(m, bm.P))) return rc;
+  hipLaunchKernelGGL(k_feat_batch, dim3(blocks_for(bm.P, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
+                     c->d_pins, s.d_pinofs + bm.begin, s.d_pair + bm.begin, bm.n, bm.P, (int64_t)1, m->a_X, m->k_in,
+                     (const int16_t*)nullptr, (int32_t*)nullptr);
+  forward(m, c->stream, bm.P, m->a_X);
+  hipLaunchKernelGGL(k_lpat_topk, dim3(blocks_for(bm.P, 256)), dim3(256), 0, c->stream, c->g, c->d_pfx, c->d_pins,
+                     s.d_pinofs + bm.begin, s.d_pair + bm.begin, s.d_chunk + bm.chunk_ofs, bm.n, bm.P, m->a_L, m->n_op,
+                     m->n_out, c->prop_k, c->prop_thr, c->d_pair_cost, c->d_pair_pat, c->d_ctr,
+                     (unsigned long long*)c->d_prop_stats);
+  HIPC(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace grh
+
+extern "C" {
+
+int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples, gr_mlp** out) {
+  if (!c || !out || hidden < 32 || hidden % 32 || max_samples < 0) return fail(c, GR_EINVAL, "bad model shape");
+  *out = nullptr;
+  (void)hipSetDevice(c->dev);
+  gr_mlp* m = new gr_mlp();
+  m->ctx = c;
+  m->n_in = 4 + 25 * c->g.Z;
+  m->k_in = (m->n_in + 31) / 32 * 32;
+  m->H = hidden;
+  m->n_out = (c->g.Z - 1) * c->g.Z2;
+  m->n_op = (m->n_out + 63) / 64 * 64;
+  size_t o = 0;
+  m->oW1 = o; o += (size_t)m->k_in * m->H;
+  m->ob1 = o; o += m->H;
+  m->oW2 = o; o += (size_t)m->H * m->H;
+  m->ob2 = o; o += m->H;
+  m->oW3 = o; o += (size_t)m->H * m->n_op;
+  m->ob3 = o; o += m->n_op;
+  m->n_params = (int)o;
+  // He-uniform initialisation of the weights (fixed seed), zero biases; padded
+  // input rows and output columns are zero
+  std::vector<float> w(o, 0.f);
+  std::mt19937_64 rng(seed);
+  auto init = [&](size_t off, int K, int N, int Kreal, int Nreal) {
+    const float a = std::sqrt(6.0f / (float)Kreal);
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int k = 0; k < K; ++k)
+      for (int n = 0; n < N; ++n) w[off + (size_t)k * N + n] = (k < Kreal && n < Nreal) ? u(rng) : 0.f;
+  };
+  init(m->oW1, m->k_in, m->H, m->n_in, m->H);
+  init(m->oW2, m->H, m->H, m->H, m->H);
+  init(m->oW3, m->H, m->n_op, m->H, m->n_out);
+  std::vector<float> loglut(kLut);
+  for (int i = 0; i < kLut; ++i) {
+    const float v = std::log(std::max(c->lut[i], 1e-30f));
+    loglut[i] = std::min(std::max(v, -40.f), 40.f) * 0.25f;
+  }
+  m->cap_samples = max_samples;
+  int rc;
+  auto bail = [&](int code) { gr_mlp_destroy(m); return code; };
+  if ((rc = dalloc(c, &m->d_master, o)) || (rc = dalloc(c, &m->d_mom, o)) || (rc = dalloc(c, &m->d_grad, o)) ||
+      (rc = dalloc(c, &m->d_loglut, kLut)) || (rc = dalloc(c, &m->d_loss, 1)) ||
+      (rc = dalloc(c, &m->W1b, (size_t)m->k_in * m->H)) || (rc = dalloc(c, &m->W1t, (size_t)m->k_in * m->H)) ||
+      (rc = dalloc(c, &m->W2b, (size_t)m->H * m->H)) || (rc = dalloc(c, &m->W2t, (size_t)m->H * m->H)) ||
+      (rc = dalloc(c, &m->W3b, (size_t)m->H * m->n_op)) || (rc = dalloc(c, &m->W3t, (size_t)m->H * m->n_op)) ||
+      (rc = dalloc(c, &m->d_X, (size_t)std::max<int64_t>(1, max_samples) * m->k_in)) ||
+      (rc = dalloc(c, &m->d_y, (size_t)std::max<int64_t>(1, max_samples))))
+    return bail(rc);
+  if (hipMemcpy(m->d_master, w.data(), o * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->d_mom, 0, o * 4) != hipSuccess ||
+      hipMemcpy(m->d_loglut, loglut.data(), kLut * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(c, GR_ENODEV, "model upload failed"));
+  refresh_bf16(m, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(c, GR_ENODEV, "model set-up failed"));
+  *out = m;
+  return 0;
+}
+
+void gr_mlp_destroy(gr_mlp* m) {
+  if (!m) return;
+  gr_ctx* c = m->ctx;
+  if (c) {
+    (void)hipStreamSynchronize(c->stream);
+    if (c->prop == m) c->prop = nullptr;
+    if (c->collect == m) c->collect = nullptr;
+  }
+  void* ps[] = {m->d_master, m->d_mom, m->d_grad, m->d_loglut, m->d_loss, m->W1b, m->W1t, m->W2b, m->W2t, m->W3b,
+                m->W3t, m->d_X, m->d_y, m->a_X, m->a_H1, m->a_H2, m->a_L, m->a_dL, m->a_dZ2, m->a_dZ1, m->a_y,
+                m->d_perm};
+  for (void* p : ps) (void)hipFree(p);
+  delete m;
+}
+
+int gr_mlp_shape(const gr_mlp* m, int32_t* s) {
+  if (!m || !s) return GR_EINVAL;
+  s[0] = m->n_in; s[1] = m->k_in; s[2] = m->H; s[3] = m->n_out; s[4] = m->n_op; s[5] = m->n_params;
+  return 0;
+}
+
+int gr_mlp_get_params(gr_mlp* m, float* out) {
+  if (!m || !out) return GR_EINVAL;
+  gr_ctx* c = m->ctx;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(out, m->d_master, (size_t)m->n_params * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gr_mlp_set_params(gr_mlp* m, const float* in) {
+  if (!m || !in) return GR_EINVAL;
+  gr_ctx* c = m->ctx;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(m->d_master, in, (size_t)m->n_params * 4, hipMemcpyHostToDevice));
+  refresh_bf16(m, c->stream);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gr_mlp_collect(gr_ctx* c, gr_mlp* m, int32_t per_batch) {
+  if (!c || (per_batch > 0 && !m) || per_batch < 0) return fail(c, GR_EINVAL, "bad collect request");
+  if (per_batch > 0 && c->prop) return fail(c, GR_ESTATE, "collect labels from exhaustive costing: clear the proposal first");
+  c->collect = per_batch > 0 ? m : nullptr;
+  c->collect_per_batch = per_batch;
+  return 0;
+}
+
+int64_t gr_mlp_samples(const gr_mlp* m) { return m ? m->n_samples : 0; }
+
+int gr_mlp_clear_samples(gr_mlp* m) {
+  if (!m) return GR_EINVAL;
+  m->n_samples = 0;
+  return 0;
+}
+
+int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss) {
+  if (!m || epochs < 0 || batch < 32 || batch % 32) return GR_EINVAL;
+  gr_ctx* c = m->ctx;
+  (void)hipSetDevice(c->dev);
+  int rc;
+  if ((rc = mlp_reserve(m, batch))) return rc;
+  const int64_t S = m->n_samples;
+  if (S < batch) return fail(c, GR_ESTATE, "fewer samples than one minibatch");
+  std::mt19937_64 rng(seed);
+  std::vector<int32_t> perm(S);
+  float last = 0.f;
+  const int steps = (int)(S / batch);
+  for (int e = 0; e < epochs; ++e) {
+    for (int64_t i = 0; i < S; ++i) perm[i] = (int32_t)i;
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIPC(c, hipMemsetAsync(m->d_loss, 0, 4, c->stream));
+    for (int st = 0; st < steps; ++st) {
+      HIPC(c, hipMemcpyAsync(m->d_perm, perm.data() + (int64_t)st * batch, sizeof(int32_t) * batch,
+                             hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, c->stream, batch, m->k_in, m->d_perm, m->d_X,
+                         m->d_y, m->a_X, m->a_y);
+      forward(m, c->stream, batch, m->a_X);
+      backward(m, c->stream, batch, batch);
+      hipLaunchKernelGGL(k_sgd, dim3(blocks_for(m->n_params, 256)), dim3(256), 0, c->stream, m->n_params,
+                         m->d_master, m->d_mom, m->d_grad, lr);
+      refresh_bf16(m, c->stream);
+      HIPC(c, hipStreamSynchronize(c->stream));  // perm chunk is a host buffer
+    }
+    float ls = 0.f;
+    HIPC(c, hipMemcpy(&ls, m->d_loss, 4, hipMemcpyDeviceToHost));
+    last = ls / (float)((int64_t)steps * batch);
+  }
+  if (loss) *loss = last;
+  return 0;
+}
+
+int gr_mlp_features(gr_ctx* c, gr_mlp* m, const int32_t* src, const int32_t* dst, int32_t n, float* out) {
+  if (!c || !m || n < 0 || (n && (!src || !dst || !out))) return fail(c, GR_EINVAL, "bad feature request");
+  if (!n) return 0;
+  (void)hipSetDevice(c->dev);
+  int32_t *ds = nullptr, *dd = nullptr;
+  uint16_t* X = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &ds, n)) || (rc = dalloc(c, &dd, n)) || (rc = dalloc(c, &X, (size_t)n * m->k_in))) {
+    (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(X);
+    return rc;
+  }
+  std::vector<uint16_t> h((size_t)n * m->k_in);
+  bool ok = hipMemcpy(ds, src, 4 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dd, dst, 4 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_feat_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
+                       ds, dd, n, X, m->k_in);
+    ok = hipStreamSynchronize(c->stream) == hipSuccess &&
+         hipMemcpy(h.data(), X, h.size() * 2, hipMemcpyDeviceToHost) == hipSuccess;
+  }
+  (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(X);
+  if (!ok) return fail(c, GR_ENODEV, "feature kernel failed");
+  for (size_t i = 0; i < h.size(); ++i) out[i] = bf2f(h[i]);
+  return 0;
+}
+
+int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
+  if (!m || n < 0 || (n && (!x || !logits))) return GR_EINVAL;
+  if (!n) return 0;
+  gr_ctx* c = m->ctx;
+  (void)hipSetDevice(c->dev);
+  int rc;
+  if ((rc = mlp_reserve(m, n))) return rc;
+  std::vector<uint16_t> h((size_t)n * m->k_in);
+  for (size_t i = 0; i < h.size(); ++i) h[i] = f2bf(x[i]);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(m->a_X, h.data(), h.size() * 2, hipMemcpyHostToDevice));
+  forward(m, c->stream, n, m->a_X);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(logits, m->a_L, (size_t)n * m->n_op * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gr_mlp_gradient(gr_mlp* m, const float* x, const int32_t* y, int32_t n, float* grad, float* loss) {
+  if (!m || n <= 0 || !x || !y || !grad) return GR_EINVAL;
+  gr_ctx* c = m->ctx;
+  (void)hipSetDevice(c->dev);
+  const int rows = (n + 31) / 32 * 32;
+  int rc;
+  if ((rc = mlp_reserve(m, rows))) return rc;
+  std::vector<uint16_t> h((size_t)rows * m->k_in, 0);
+  for (size_t i = 0; i < (size_t)n * m->k_in; ++i) h[i] = f2bf(x[i]);
+  std::vector<int32_t> yy(rows, -1);
+  for (int i = 0; i < n; ++i) yy[i] = y[i];
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(m->a_X, h.data(), h.size() * 2, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(m->a_y, yy.data(), 4 * (size_t)rows, hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(m->d_loss, 0, 4));
+  forward(m, c->stream, rows, m->a_X);
+  backward(m, c->stream, rows, n);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(grad, m->d_grad, (size_t)m->n_params * 4, hipMemcpyDeviceToHost));
+  float ls = 0.f;
+  HIPC(c, hipMemcpy(&ls, m->d_loss, 4, hipMemcpyDeviceToHost));
+  if (loss) *loss = ls / (float)n;
+  return 0;
+}
+
+int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
+  if (!c) return fail(c, GR_EINVAL, "null context");
+  if (m && (k < 1 || k > kMaxTopK || m->ctx != c)) return fail(c, GR_EINVAL, "k must be 1..8 and the model this context's");
+  if (m && c->collect) return fail(c, GR_ESTATE, "stop sample collection before routing with the proposal");
+  (void)hipSetDevice(c->dev);
+  if (!c->d_prop_stats) {
+    int rc;
+    if ((rc = dalloc(c, &c->d_prop_stats, 2))) return rc;
+    HIPC(c, hipMemset(c->d_prop_stats, 0, 16));
+  }
+  c->prop = m;
+  c->prop_k = k;
+  c->prop_thr = threshold;
+  return 0;
+}
+
+int gr_proposal_stats(gr_ctx* c, int64_t* pairs, int64_t* fallbacks) {
+  if (!c || !pairs || !fallbacks) return fail(c, GR_EINVAL, "null argument");
+  *pairs = *fallbacks = 0;
+  if (!c->d_prop_stats) return 0;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  int64_t v[2];
+  HIPC(c, hipMemcpy(v, c->d_prop_stats, 16, hipMemcpyDeviceToHost));
+  HIPC(c, hipMemset(c->d_prop_stats, 0, 16));
+  *pairs = v[0];
+  *fallbacks = v[1];
+  return 0;
+}
+
+}  // extern "C"

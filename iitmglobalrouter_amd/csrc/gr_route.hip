@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/iitmgr.h"
+#include "gr_internal.h"
 #include "gr_kernels.h"
 
 using namespace grd;
@@ -37,98 +38,15 @@ void build_lut(int model, float* lut) {
   }
 }
 
-// Net size classes of the tree stage: 2 pins; up to 8, 16, 32, 64 pins
-// (one group of that many lanes per net); more (one workgroup per net).
-constexpr int kNumCls = 6;
-constexpr int kClsLarge = 5;
-constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, 1 << 30};
-int net_class(int64_t d) {
-  int k = 0;
-  while (d > kClsMax[k]) ++k;
-  return k;
-}
-
-struct BatchMeta {
-  int32_t begin, n;       // schedule positions [begin, begin+n)
-  int64_t P;              // pin pairs in the batch
-  int32_t chunk_ofs;      // first entry of the batch in the pair-chunk table
-  int32_t cls_ofs[kNumCls], cls_n[kNumCls];
-  int32_t max_large;      // most pins of a large net in the batch
-  int64_t max_large_P;    // most pin pairs of a large net in the batch
-};
-
-struct Sched {
-  std::vector<BatchMeta> meta;
-  int32_t n_pos = 0;
-  int64_t slots = 0, max_P = 0;
-  int64_t* d_pinofs = nullptr;  // position -> first pin of the net
-  int32_t* d_deg = nullptr;     // position -> pin count
-  int64_t* d_slot = nullptr;
-  int64_t* d_pair = nullptr;
-  int32_t* d_list = nullptr;
-  int32_t* d_chunk = nullptr;
-  int64_t* d_edges = nullptr;
-  int32_t* d_lpat = nullptr;
-  float* d_cost = nullptr;
-  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0;
-  hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
-  bool graph_failed = false;
-};
-
-struct EvPair { hipEvent_t a, b; int kind; };
-
-std::string g_err;
-
 }  // namespace
 
-struct gr_ctx {
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  Geo g{};
-  int model = 0, quirk = 1, M = 0;
-  int64_t cs = 0;
-  float lut[kLut];
-  int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;
-  float *d_pfx = nullptr, *d_lut = nullptr;
-  int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
-  size_t lds_max = 65536;
-  int32_t n_nets = 0;
-  std::vector<int64_t> h_pin_pfx;
-  int64_t* d_pin_pfx = nullptr;
-  int32_t* d_pins = nullptr;
-  Sched sched, single;
-  float* d_pair_cost = nullptr;
-  int16_t* d_pair_pat = nullptr;
-  int64_t pair_cap = 0, last_P = 0;
-  bool timing = false;
-  std::vector<EvPair> evs;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  std::string err;
-};
+namespace grh {
+std::string g_err;
+}  // namespace grh
+
+using namespace grh;
 
 namespace {
-
-int fail(gr_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg; else g_err = msg;
-  return code;
-}
-
-#define HIPC(c, x)                                                                      \
-  do {                                                                                  \
-    hipError_t e_ = (x);                                                                \
-    if (e_ != hipSuccess)                                                               \
-      return fail((c), GR_ENODEV, std::string(#x " failed: ") + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class T>
-int dalloc(gr_ctx* c, T** p, size_t n) {
-  if (*p) { (void)hipFree(*p); *p = nullptr; }
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) { *p = nullptr; return fail(c, GR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
-  return 0;
-}
 
 hipEvent_t take_event(gr_ctx* c) {
   if (c->ev_used == c->ev_pool.size()) {
@@ -278,7 +196,11 @@ LpatFn lane_kernel(int Z) {
 int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   const BatchMeta& m = s.meta[b];
   if (m.n == 0) return 0;
-  if (m.P > 0) {
+  if (m.P > 0 && c->prop) {
+    KTimer t(c, GR_K_LPAT);
+    const int rc = mlp_enqueue_costing(c, s, m);
+    if (rc) return rc;
+  } else if (m.P > 0) {
     KTimer t(c, GR_K_LPAT);
     LpatFn fn = lane_kernel(c->g.Z);
     if (fn) {
@@ -290,6 +212,10 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
                          c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     }
+  }
+  if (m.P > 0 && c->collect) {
+    const int rc = mlp_enqueue_collect(c, s, m);
+    if (rc) return rc;
   }
   TreeArgs A;
   A.g = c->g;
@@ -505,7 +431,7 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
   (void)hipSetDevice(c->dev);
   Sched& s = c->sched;
   const bool whole = first == 0 && count == (int32_t)s.meta.size() && count > 0;
-  if (whole && !c->timing && !s.graph_failed) {
+  if (whole && !c->timing && !c->prop && !c->collect && !s.graph_failed) {
     if (!s.exec) {
       // capture the whole pass once: kernels only (no memset nodes), fixed
       // arguments; the batch stamp comes from the device counter

@@ -169,6 +169,48 @@ int gr_lut_size(void);
 int gr_enable_timing(gr_ctx* ctx, int on);
 int gr_kernel_stats(gr_ctx* ctx, int64_t* launches, double* total_ms);
 
+/* ---- learned proposal (no counterpart in the reference) ----
+ *
+ * A 3-layer MLP (n_in -> hidden -> hidden -> n_out, ReLU) scores the L-pattern
+ * candidates of a pin pair so that only its top-k are costed.  n_in = 4 + 25*Z
+ * features per pair: dx/64, dy/64, z_src, z_dst, then for every layer the log
+ * cell cost at a 5x5 lattice spanning the pair's bounding box.  n_out =
+ * (Z-1)*Z2 candidates (patterns with first layer a >= 1).  Weights are bf16 on
+ * MFMA with fp32 accumulation and an fp32 master copy; training runs on the
+ * GPU with the library's own forward and backward kernels on samples captured
+ * from exhaustive routing (the chosen pattern is the label).
+ */
+typedef struct gr_mlp gr_mlp;
+
+int gr_mlp_create(gr_ctx* ctx, int32_t hidden, uint64_t seed, int32_t max_samples, gr_mlp** out);
+void gr_mlp_destroy(gr_mlp* m);
+/* n_in, k_in (n_in padded to 32), hidden, n_out, n_out padded (64), parameter count */
+int gr_mlp_shape(const gr_mlp* m, int32_t* shape6);
+/* fp32 master parameters: W1 [k_in x hidden], b1 [hidden], W2 [hidden x hidden], b2,
+ * W3 [hidden x n_out_pad], b3 [n_out_pad], row-major, concatenated. */
+int gr_mlp_get_params(gr_mlp* m, float* out);
+int gr_mlp_set_params(gr_mlp* m, const float* in);
+/* While per_batch > 0, every routed batch appends up to per_batch samples
+ * (features at the state before its commit, label = exhaustive choice). */
+int gr_mlp_collect(gr_ctx* ctx, gr_mlp* m, int32_t per_batch);
+int64_t gr_mlp_samples(const gr_mlp* m);
+int gr_mlp_clear_samples(gr_mlp* m);
+/* SGD with momentum 0.9 over the samples; *loss = mean cross-entropy of the last epoch. */
+int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss);
+/* Features of explicit pairs (src/dst cell encodings) at the current state: n x k_in. */
+int gr_mlp_features(gr_ctx* ctx, gr_mlp* m, const int32_t* src, const int32_t* dst, int32_t n, float* out);
+/* Forward pass on host rows x (n x k_in, rounded to bf16 on upload) -> logits (n x n_out_pad). */
+int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits);
+/* Gradient of the mean cross-entropy over (x, y) w.r.t. all parameters; no update. */
+int gr_mlp_gradient(gr_mlp* m, const float* x, const int32_t* y, int32_t n, float* grad, float* loss);
+/* Route with the model's top-k candidates (1 <= k <= 8), falling back to full
+ * enumeration when the top-1 softmax probability is below threshold.  m = NULL
+ * restores exhaustive costing.  The chosen pattern is the lexicographic
+ * minimum of (cost, p) over the costed candidates. */
+int gr_set_proposal(gr_ctx* ctx, gr_mlp* m, int32_t k, float threshold);
+/* Pairs costed and pairs that fell back since the last call (counters reset). */
+int gr_proposal_stats(gr_ctx* ctx, int64_t* pairs, int64_t* fallbacks);
+
 /* ---- host side of the path (no device work) ---- */
 
 /*

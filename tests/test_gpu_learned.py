@@ -1,0 +1,171 @@
+"""GPU tests of the learned proposal: MFMA forward and gradients against the
+numpy CPU reference, features, top-k costing against the oracle, and what the
+trained network gains over the untrained one."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _router(gr, X=64, Y=64, Z=10, kind=2, seed=3, nets=1500):
+    cap = gr.gen_grid(kind, seed, X, Y, Z)
+    pfx, pins = gr.gen_nets(kind, seed, X, Y, Z, nets)
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    return r, cap, pfx, pins, ofst, ids
+
+
+def _rand_params(m, rng, scale=1.0):
+    p = rng.standard_normal(m.n_params).astype(np.float32) * 0.05 * scale
+    return p
+
+
+def test_forward_matches_cpu(gr, gpu):
+    from oracle import mlp_ref
+    r, *_ = _router(gr)
+    m = gr.Model(r, hidden=256, seed=1)
+    rng = np.random.default_rng(0)
+    p = _rand_params(m, rng)
+    m.set_params(p)
+    x = rng.standard_normal((300, m.k_in)).astype(np.float32)
+    got = m.forward(x)
+    want, _ = mlp_ref.forward(mlp_ref.unpack(p, r.Z), x)
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 2e-2 * scale, np.abs(got - want).max()
+    assert np.abs(got - want).mean() <= 2e-3 * scale
+
+
+def test_gradient_matches_cpu(gr, gpu):
+    from oracle import mlp_ref
+    r, *_ = _router(gr)
+    m = gr.Model(r, hidden=256, seed=2)
+    rng = np.random.default_rng(1)
+    p = _rand_params(m, rng, 2.0)
+    m.set_params(p)
+    x = rng.standard_normal((100, m.k_in)).astype(np.float32)
+    y = rng.integers(0, m.n_out, 100).astype(np.int32)
+    g, loss = m.gradient(x, y)
+    gw, lw = mlp_ref.gradient(mlp_ref.unpack(p, r.Z), x, y, m.n_out)
+    assert abs(loss - lw) <= 1e-3 * max(1.0, abs(lw))
+    G, W = mlp_ref.unpack(g, r.Z), mlp_ref.unpack(gw, r.Z)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        err = np.linalg.norm(G[k] - W[k]) / max(np.linalg.norm(W[k]), 1e-12)
+        assert err < 3e-2, (k, err)
+
+
+def test_features_match_cpu(gr, orc, gpu):
+    from oracle import mlp_ref
+    r, cap, pfx, pins, ofst, ids = _router(gr, X=96, Y=80, nets=2500)
+    m = gr.Model(r, seed=0)
+    for b in range(3):  # some congestion first
+        r.route_batch(ids[ofst[b]:ofst[b + 1]])
+    rem = cap - r.demand()
+    rng = np.random.default_rng(4)
+    src = rng.integers(0, len(pins), 200)
+    dst = rng.integers(0, len(pins), 200)
+    s, d = pins[src], pins[dst]
+    got = m.features(s, d)
+    want = mlp_ref.features(r.X, r.Y, r.Z, rem, r.lut(), s, d)
+    # equal up to one bf16 ulp (host logf vs numpy log of the same LUT entry)
+    assert np.all(np.abs(got - want) <= np.abs(want) * 2.0 ** -7 + 1e-6)
+
+
+def test_always_fallback_is_exhaustive(gr, gpu):
+    a, cap, pfx, pins, ofst, ids = _router(gr, X=96, Y=96, nets=2500)
+    b = gr.Router(96, 96, 10, cap)
+    b.set_nets(pfx, pins)
+    m = gr.Model(b, seed=5)
+    b.set_proposal(m, k=4, threshold=2.0)  # top-1 probability never reaches 2
+    for i in range(len(ofst) - 1):
+        e1 = a.route_batch(ids[ofst[i]:ofst[i + 1]])
+        e2 = b.route_batch(ids[ofst[i]:ofst[i + 1]])
+        for u, v in zip(e1, e2):
+            assert np.array_equal(np.asarray(u).view(np.int32), np.asarray(v).view(np.int32))
+    assert np.array_equal(a.demand(), b.demand())
+    pairs, fb = b.proposal_stats()
+    assert pairs > 0 and fb == pairs
+
+
+def test_topk_choice_matches_oracle(gr, orc, gpu):
+    """With no fallback, each pair's pattern is the (cost, p) minimum over the
+    model's top-k candidates, costed by the oracle at the same state."""
+    X, Y, Z = 96, 96, 10
+    r, cap, pfx, pins, ofst, ids = _router(gr, X=X, Y=Y, nets=2500)
+    m = gr.Model(r, seed=7)
+    rng = np.random.default_rng(3)
+    m.set_params(_rand_params(m, rng, 4.0))
+    o = orc.Oracle(X, Y, Z, cap)
+    k = 3
+    r.set_proposal(m, k=k, threshold=0.0)
+    Z2 = (Z + 1) // 2
+    for b in range(2):
+        batch = ids[ofst[b]:ofst[b + 1]]
+        src, dst = [], []
+        for nid in batch:
+            p = pins[pfx[nid]:pfx[nid + 1]]
+            for j in range(1, len(p)):
+                for kk in range(j):
+                    src.append(p[j]); dst.append(p[kk])
+        logits = m.forward(m.features(np.array(src), np.array(dst)))[:, :m.n_out]
+        o.set_rem(cap - r.demand())
+        r.route_batch(batch)
+        cost, pat = r.pair_costs()
+        for i in range(len(src)):
+            order = sorted(range(m.n_out), key=lambda q: (-logits[i, q], q))[:k]
+            best = None
+            for q in order:
+                c = o.pattern_cost(src[i], dst[i], q + Z2)
+                if c is not None and (best is None or (np.float32(c), q + Z2) < best):
+                    best = (np.float32(c), q + Z2)
+            if best is None:
+                pp, cc = o.best_pattern(src[i], dst[i])
+                best = (np.float32(cc), pp)
+            assert (pat[i], cost[i]) == (best[1], best[0]), i
+    r.set_proposal(None)
+
+
+def test_training_gains(gr, gpu):
+    """Samples from exhaustive routing of 4 seeds; the trained model routes a
+    held-out seed with top-4 candidates closer to exhaustive than the
+    untrained model, and its loss falls."""
+    X, Y, Z, nets = 200, 200, 10, 6000
+    cap = gr.gen_grid(2, 0, X, Y, Z)
+    r = gr.Router(X, Y, Z, cap)
+    m = gr.Model(r, seed=11, max_samples=200000)
+
+    def load(seed):
+        pfx, pins = gr.gen_nets(2, seed, X, Y, Z, nets)
+        ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+        r.set_nets(pfx, pins)
+        r.set_schedule(ofst, ids)
+
+    def route_pass():
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+        return gr.route_cost(cap, r.demand())["total"]
+
+    m.collect(4000)
+    for seed in (1, 2, 3, 4):
+        load(seed)
+        route_pass()
+    m.collect(0)
+    assert m.samples() > 20000
+    load(0)
+    exhaustive = route_pass()
+    r.set_proposal(m, k=4, threshold=0.0)
+    untrained = route_pass()
+    r.set_proposal(None)
+    l0 = m.train(epochs=1, batch=512, lr=0.05, seed=1)
+    l1 = m.train(epochs=6, batch=512, lr=0.05, seed=2)
+    print(f"loss {l0:.4f} -> {l1:.4f}", flush=True)
+    assert l1 < l0
+    r.set_proposal(m, k=4, threshold=0.0)
+    trained = route_pass()
+    r.set_proposal(None)
+    gap_u = (untrained - exhaustive) / exhaustive
+    gap_t = (trained - exhaustive) / exhaustive
+    print(f"route cost exhaustive {exhaustive} untrained {untrained} ({gap_u:.4%}) trained {trained} ({gap_t:.4%})",
+          flush=True)
+    assert gap_t < gap_u
