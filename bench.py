@@ -72,6 +72,70 @@ def cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, budget_s):
                       f"C++ oracle with OpenMP over nets, {dt:.1f} s"}
 
 
+def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
+    """Learned proposal on the same generator: samples from exhaustive routing of
+    seeds 1..train_seeds train the MLP on the GPU; seed 0 is then routed with the
+    model's top-k candidates (fallback to full enumeration below 0.6 top-1
+    probability) and judged against exhaustive costing of the same seed."""
+    t0 = time.perf_counter()
+    m = gr.Model(r, hidden=256, seed=0, max_samples=args.train_seeds * 70 * args.train_per_batch)
+    m.collect(args.train_per_batch)
+    for s in range(1, args.train_seeds + 1):
+        pf, pn = gr.gen_nets(args.workload, s, X, Y, Z, n_req)
+        of, ids = gr.partition_nets(X, Y, Z, pf, pn, 4)
+        r.set_nets(pf, pn)
+        r.set_schedule(of, ids)
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+    m.collect(0)
+    t1 = time.perf_counter()
+    losses = [m.train(epochs=1, batch=1024, lr=0.05, seed=e) for e in range(args.train_epochs)]
+    t2 = time.perf_counter()
+    pfx, pins, ofst, ids = seed0
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    exhaustive = gr.route_cost(cap, r.demand())
+    P = float((np.diff(pfx) * (np.diff(pfx) - 1) // 2).sum())
+    flop_pair = 2.0 * (m.k_in * m.hidden + m.hidden * m.hidden + m.hidden * m.n_out_pad)
+    out = {"train": {"seeds": args.train_seeds, "samples": int(m.samples()), "epochs": args.train_epochs,
+                     "loss_per_epoch": [round(v, 4) for v in losses], "collect_s": round(t1 - t0, 2),
+                     "train_s": round(t2 - t1, 2)},
+           "exhaustive_route_cost": exhaustive}
+    for k in (4, 8):
+        r.set_proposal(m, k=k, threshold=0.6)
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+        r.proposal_stats()
+        r.enable_timing(True)
+        steps = 2
+        ta = time.perf_counter()
+        for _ in range(steps):
+            r.reset_demand()
+            r.route_schedule()
+        r.sync()
+        tb = time.perf_counter()
+        n, ms = r.kernel_stats()
+        r.enable_timing(False)
+        pairs, fb = r.proposal_stats()
+        cost = gr.route_cost(cap, r.demand())
+        r.set_proposal(None)
+        ms_l = (tb - ta) * 1e3 / steps
+        cost_ms = float(ms[0]) / steps  # features + MLP + top-k costing
+        out[f"k{k}"] = {
+            "nets_per_s": round(n_nets / (ms_l / 1e3), 1), "ms_per_step": round(ms_l, 3),
+            "vs_plain": round(plain_ms / ms_l, 4), "route_cost": cost,
+            "route_cost_gap": round((cost["total"] - exhaustive["total"]) / exhaustive["total"], 6),
+            "fallback_rate": round(fb / max(pairs, 1), 4), "costing_ms_per_step": round(cost_ms, 3),
+            "mlp_TFLOPs_lower_bound": round(P * flop_pair / (cost_ms / 1e3) / 1e12, 2)}
+    m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +146,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
+    ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
+    ap.add_argument("--train-seeds", type=int, default=20)
+    ap.add_argument("--train-per-batch", type=int, default=1500)
+    ap.add_argument("--train-epochs", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the routing path runs on a single MI355X "
@@ -154,7 +222,17 @@ def main():
         ach = kbytes[dom] / n_b / avg_s / 1e9
         roof = {"kernel": KNAMES[dom], "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                "avg_launch_us": round(avg_s * 1e6, 2)}
+                "avg_launch_us": round(avg_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": round(kbytes[dom] / n_b)}
+        # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+        # (calibrated on a copy kernel) for this workload; see profiles/
+        prof = os.path.join(ROOT, "profiles", f"pmc_traffic_workload{args.workload}.json")
+        if os.path.exists(prof):
+            with open(prof) as f:
+                roof["traffic"] = json.load(f)["per_launch"][KNAMES[dom]]["traffic_bytes"]
+    learned = None
+    if args.learned:
+        learned = learned_eval(gr, r, X, Y, Z, cap, W["nets"], args, n_nets, ms_step, (pfx, pins, ofst, ids))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, args.cpu_seconds)
@@ -170,6 +248,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kstats,
+        "learned": learned,
         "path_bytes_per_step": bytes_path,
         "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
     }

@@ -60,6 +60,8 @@ struct EvPair { hipEvent_t a, b; int kind; };
 struct gr_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // large-net trees, joined back before the refresh
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   grd::Geo g{};
   int model = 0, quirk = 1, M = 0;
   int64_t cs = 0;

@@ -248,6 +248,15 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       blocks += (C.n[k] + per_block - 1) / per_block;
     }
     C.block0[5] = blocks;
+    // small and large nets are independent: with both present, the large-net
+    // workgroups run on a second stream (a parallel branch of the graph)
+    const bool fork = blocks && m.cls_n[kClsLarge] && !c->timing;
+    hipStream_t lst = c->stream;
+    if (fork) {
+      HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+      HIPC(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+      lst = c->stream2;
+    }
     if (blocks) hipLaunchKernelGGL(k_tree_small, dim3(blocks), dim3(256), 0, c->stream, A, C);
     if (m.cls_n[kClsLarge]) {
       A.list = s.d_list + m.cls_ofs[kClsLarge];
@@ -256,7 +265,11 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 32) / 4;
       const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
       const size_t lds = (size_t)cap * 32 + (size_t)scap * 4;
-      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(1024), lds, c->stream, A, cap, scap);
+      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(1024), lds, lst, A, cap, scap);
+    }
+    if (fork) {
+      HIPC(c, hipEventRecord(c->ev_join, c->stream2));
+      HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
   }
   {
@@ -325,7 +338,11 @@ int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
   build_lut(c->model, c->lut);
   int rc = 0;
   auto bail = [&](int code) { gr_destroy(c); return code; };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(nullptr, GR_ENODEV, "stream"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_stamp, cs)) ||
       (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
@@ -366,6 +383,10 @@ void gr_destroy(gr_ctx* c) {
   free_sched(c->sched);
   free_sched(c->single);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
