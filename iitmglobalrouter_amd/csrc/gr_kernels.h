@@ -385,7 +385,44 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
     if (ne >= n - 1) break;
     for (int i = tid; i < n; i += gsize) best[i] = ~0ull;
     sync();
-    {
+    if (cost == gcost) {
+      // costs did not fit in LDS: each thread streams a contiguous chunk of
+      // pairs from global memory.  The row root's running minimum and that of
+      // the current run of equal column roots stay in registers and go to LDS
+      // (atomicMin) only when the root changes.
+      const uint32_t chunk = (P + gsize - 1) / gsize;
+      uint32_t o = (uint32_t)tid * chunk;
+      const uint32_t oe = o + chunk < P ? o + chunk : P;
+      if (o < oe) {
+        int j, k;
+        pair_decode(o, j, k);
+        int cj = root[j], ck = -1;
+        unsigned long long mj = ~0ull, mk = ~0ull;
+        for (; o < oe; ++o) {
+          const int rk = root[k];
+          if (rk != cj) {
+            const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+            mj = key < mj ? key : mj;
+            if (rk != ck) {
+              if (mk != ~0ull) atomicMin(&best[ck], mk);
+              ck = rk;
+              mk = key;
+            } else {
+              mk = key < mk ? key : mk;
+            }
+          }
+          if (++k == j) {
+            if (mj != ~0ull) atomicMin(&best[cj], mj);
+            mj = ~0ull;
+            ++j;
+            k = 0;
+            if (j < n) cj = root[j];
+          }
+        }
+        if (mj != ~0ull) atomicMin(&best[cj], mj);
+        if (mk != ~0ull) atomicMin(&best[ck], mk);
+      }
+    } else {
       // node-centric: S adjacent threads scan node i's n-1 pairs (rows and
       // columns of the triangle) into a register min; one atomic per node.
       const int S = (n * 8 <= gsize) ? 8 : (n * 4 <= gsize) ? 4 : (n * 2 <= gsize) ? 2 : 1;

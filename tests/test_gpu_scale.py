@@ -1,0 +1,84 @@
+"""GPU parity at scale: nets of hundreds of pins (pair costs streamed from
+global memory in the tree stage) and the full 2000x2000x10 / 1.5M-net
+workload, where sampled batches are re-routed by the oracle from the GPU's own
+state and must match bit-exactly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_f32(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.int32), np.asarray(b, np.float32).view(np.int32))
+
+
+def _enc(X, Y, z, x, y):
+    return z * X * Y + (x * Y + y if z & 1 else y * X + x)
+
+
+def test_huge_nets(gr, orc, gpu):
+    X, Y, Z = 256, 256, 10
+    cap = gr.gen_grid(2, 9, X, Y, Z)
+    rng = np.random.default_rng(9)
+    nets = []
+    for npins in (900, 620, 333, 170, 90, 40, 7, 2):
+        s = set()
+        while len(s) < npins:
+            s.add(_enc(X, Y, int(rng.integers(0, 2)), int(rng.integers(0, X)), int(rng.integers(0, Y))))
+        nets.append(sorted(s))
+    pfx = np.cumsum([0] + [len(n) for n in nets]).astype(np.int64)
+    pins = np.concatenate([np.array(n, np.int32) for n in nets])
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    o = orc.Oracle(X, Y, Z, cap)
+    o.set_nets(pfx, pins)
+    # every box spans most of the grid: one net per batch, largest first
+    for nid in range(len(nets)):
+        print(f"net {nid} pins {len(nets[nid])}", flush=True)
+        e1, l1, c1 = r.route_batch([nid])
+        e2, l2, c2 = o.route_batch([nid])
+        pc1, pp1 = r.pair_costs()
+        pc2, pp2 = o.pair_costs()
+        assert _same_f32(pc1, pc2) and np.array_equal(pp1, pp2)
+        assert np.array_equal(e1, e2) and np.array_equal(l1, l2) and _same_f32(c1, c2)
+    assert np.array_equal(r.demand(), o.demand())
+
+
+def test_workload3_sampled_batches(gr, orc, gpu):
+    X, Y, Z = 2000, 2000, 10
+    cap = gr.gen_grid(3, 0, X, Y, Z)
+    pfx, pins = gr.gen_nets(3, 0, X, Y, Z, 1500000)
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    nb = len(ofst) - 1
+    deg = np.diff(pfx)
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    o = orc.Oracle(X, Y, Z, cap)
+    o.set_nets(pfx, pins)
+    done = 0
+    slot0 = np.concatenate([[0], np.cumsum(deg[ids])])
+    for b in (1, nb // 2, nb - 1):
+        print(f"workload3 batch {b} of {nb}: {ofst[b + 1] - ofst[b]} nets", flush=True)
+        r.route_schedule(done, b - done)
+        r.sync()
+        o.set_rem(cap - r.demand())
+        r.route_schedule(b, 1)
+        done = b + 1
+        e1, l1, c1 = r.schedule_outputs()
+        batch = ids[ofst[b]:ofst[b + 1]]
+        e2, l2, c2 = o.route_batch(batch)
+        s0, s1 = slot0[ofst[b]], slot0[ofst[b + 1]]
+        pc1, pp1 = r.pair_costs()
+        pc2, pp2 = o.pair_costs()
+        assert _same_f32(pc1, pc2) and np.array_equal(pp1, pp2), b
+        assert np.array_equal(e1[s0:s1], e2) and np.array_equal(l1[s0:s1], l2), b
+        assert _same_f32(c1[ofst[b]:ofst[b + 1]], c2), b
+        assert np.array_equal(r.demand(), o.demand()), b
+    # whole pass: every net has deg-1 edges and one sentinel in its slots
+    r.reset_demand()
+    r.route_schedule()
+    e, l, c = r.schedule_outputs()
+    last = slot0[1:] - 1
+    assert np.all(e[last] == -1) and np.all(l[last] == -1)
+    assert int((e >= 0).sum()) == int((deg - 1).sum())
