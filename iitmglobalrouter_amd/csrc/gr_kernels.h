@@ -338,11 +338,25 @@ __device__ void tree2_commit(const TreeArgs& A, int pos) {
   A.out_cost[A.cost_base + pos] = 0.0f + A.pair_cost[A.pair[pos]];
 }
 
-// Root-pair Boruvka (gmst_util.hpp:164-341) with the lowest-index tie contract,
-// one round at a time over a group of gsize threads (a wavefront or a
-// workgroup; node r is handled by thread r mod gsize):
-//   min phase    per component root, min (cost, o) over outgoing pairs
-//                (LDS 64-bit atomicMin; roots are kept flattened)
+// Lock-free DSU hooking on LDS: the smaller root is always hooked under the
+// larger (as gmst_util.hpp:314-325 does serially), so every component's root
+// stays its largest member whatever the order of concurrent unions.  A failed
+// CAS means another union moved a; retrying from the new roots always ends.
+__device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
+  volatile int* vp = par;
+  for (int guard = 0; guard < bound; ++guard) {
+    while (vp[a] != a) a = vp[a];
+    while (vp[b] != b) b = vp[b];
+    if (a == b) return;
+    if (a > b) { const int t = a; a = b; b = t; }
+    if (atomicCAS(&par[a], a, b) == a) return;
+  }
+}
+
+// Root-pair Boruvka (gmst_util.hpp:164-341) with the lowest-index tie contract
+// for one net on one workgroup (node r handled by thread r mod gsize):
+//   min phase    per component root, min (cost, o) over outgoing pairs; the
+//                pair costs are staged in LDS when they fit, else streamed
 //   partner      root of the other endpoint of that pair
 //   accept       the reference processes candidates serially in ascending
 //                root order and drops one whose roots already joined.  With
@@ -351,18 +365,13 @@ __device__ void tree2_commit(const TreeArgs& A, int pos) {
 //                mutual pair is dropped: the serial outcome, computed in
 //                parallel.  Accepted edges (root, partner) are written in
 //                ascending root order.
-//   merge        candidate links, pointer jumping to each tree's root, new
-//                root = max index of the merged component (what hooking the
-//                smaller root under the larger yields, gmst_util.hpp:314-325).
+//   merge        accepted edges are united by lock-free hooking; roots are
+//                then flattened (root = the component's largest member).
 // Rounds repeat until deg-1 edges (the reference's feedback loop,
 // mst_util.hpp:298-314, ends in the same state).
-template <bool WAVE>
 __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int* root, unsigned long long* best,
-                            int* partner, int* link, int2* edges, float* ecost, int* wcnt, int* sh, float* scost,
+                            int* partner, int* par, int2* edges, float* ecost, int* wcnt, int* sh, float* scost,
                             int scap) {
-  auto sync = [&]() {
-    if (WAVE) wave_sync(); else __syncthreads();
-  };
   const int64_t pb = A.pinofs[pos];
   const int n = A.deg[pos];
   const float* gcost = A.pair_cost + A.pair[pos];
@@ -377,14 +386,17 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
   }
   for (int i = tid; i < n; i += gsize) root[i] = i;
   if (tid == 0) sh[0] = 0;
-  sync();
+  __syncthreads();
   // Each round adds at least one edge (the globally smallest key is a mutual
   // pick), so n rounds always suffice; the bound only guards termination.
   for (int round = 0; round < n; ++round) {
     const int ne = sh[0];
     if (ne >= n - 1) break;
-    for (int i = tid; i < n; i += gsize) best[i] = ~0ull;
-    sync();
+    for (int i = tid; i < n; i += gsize) {
+      best[i] = ~0ull;
+      par[i] = i;
+    }
+    __syncthreads();
     if (cost == gcost) {
       // costs did not fit in LDS: each thread streams a contiguous chunk of
       // pairs from global memory.  The row root's running minimum and that of
@@ -448,7 +460,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
         if (i < n && part == 0 && m != ~0ull) atomicMin(&best[ri], m);
       }
     }
-    sync();
+    __syncthreads();
     for (int r = tid; r < n; r += gsize) {
       int pr = -1;
       if (root[r] == r && best[r] != ~0ull) {
@@ -458,66 +470,42 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
       }
       partner[r] = pr;
     }
-    sync();
-    // accept + ordered slots
+    __syncthreads();
+    // accept, ordered slots, union
     const int iters = (n + gsize - 1) / gsize;
     for (int it = 0; it < iters; ++it) {
       const int r = it * gsize + tid;
       bool acc = false;
+      int pr = -1;
       if (r < n) {
-        const int pr = partner[r];
+        pr = partner[r];
         acc = pr >= 0 && !(partner[pr] == r && r > pr);
-        link[r] = acc ? pr : r;
       }
       const unsigned long long m = __ballot(acc);
       if (lane == 0) wcnt[it * nwv + wv] = __popcll(m);
-      sync();
+      __syncthreads();
       if (acc) {
         int off = ne;
         for (int q = 0; q < it * nwv + wv; ++q) off += wcnt[q];
         off += __popcll(m & ((1ull << lane) - 1ull));
-        edges[off] = make_int2(r, partner[r]);
+        edges[off] = make_int2(r, pr);
         ecost[off] = cost[(uint32_t)best[r]];
+        dsu_union(par, r, pr, 4 * n);
       }
     }
-    sync();
+    __syncthreads();
     if (tid == 0) {
       int tot = ne;
       for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
       sh[0] = tot;
     }
-    // pointer jumping over the candidate forest (roots only); depth < 2^32
-    for (int jump = 0; jump < 32; ++jump) {
-      int changed = 0;
-      for (int r = tid; r < n; r += gsize) {
-        if (root[r] != r) continue;
-        const int l1 = link[r], l2 = link[l1];
-        if (l2 != l1) { link[r] = l2; changed = 1; }
-      }
-      bool any;
-      if (WAVE) {
-        any = __ballot(changed) != 0ull;
-      } else {
-        any = __syncthreads_or(changed) != 0;
-      }
-      if (!any) break;
-      sync();
-    }
-    sync();
-    // new root of each merged component = its largest old root (= max pin index)
-    for (int r = tid; r < n; r += gsize)
-      if (root[r] == r) partner[r] = r;
-    sync();
-    for (int r = tid; r < n; r += gsize)
-      if (root[r] == r) atomicMax(&partner[link[r]], r);
-    sync();
+    // flatten: every node's root becomes its component's largest member
     for (int i = tid; i < n; i += gsize) {
-      const int ro = root[i];
-      best[i] = (unsigned long long)partner[link[ro]];  // staged: reads finish before root[] changes
+      int x = root[i];
+      while (par[x] != x) x = par[x];
+      root[i] = x;
     }
-    sync();
-    for (int i = tid; i < n; i += gsize) root[i] = (int)best[i];
-    sync();
+    __syncthreads();
   }
   const int64_t sb = A.slot[pos];
   const int32_t* npins = A.pins + pb;
@@ -535,8 +523,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
 // a group of G lanes per net, node i on lane i, 64/G nets per wavefront.
 // Groups of one wavefront advance in lockstep (wave-wide syncs); a group whose
 // net is complete idles until every group of the wave is.  Phases and their
-// justification are those of tree_commit; the pointer jumping runs a fixed 6
-// doublings (depth < 64).
+// justification are those of tree_commit.
 struct GroupLds {
   int root[64];
   unsigned long long best[64];
@@ -575,7 +562,10 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     const bool act = ne < n - 1;
     if (__ballot(act) == 0ull) break;
     const bool me = act && tid < n;
-    if (me) best[tid] = ~0ull;
+    if (me) {
+      best[tid] = ~0ull;
+      link[tid] = tid;  // DSU parent for this round's unions
+    }
     wave_sync();
     if (me) {
       const int ri = root[tid];
@@ -599,27 +589,20 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     if (me) partner[tid] = pr;
     wave_sync();
     const bool acc = me && pr >= 0 && !(partner[pr] == tid && tid > pr);
-    if (me) link[tid] = acc ? pr : tid;
     const unsigned long long gm = __ballot(acc) & gbits;
     if (acc) {
       const int off = ne + __popcll(gm & ((1ull << lane) - 1ull));
       edges[off] = make_int2(tid, pr);
       ecost[off] = cost[(uint32_t)best[tid]];
+      dsu_union(link, tid, pr, 4 * G);
     }
     ne += __popcll(gm);
     wave_sync();
-#pragma unroll
-    for (int jump = 0; jump < 6; ++jump) {
-      if (me && root[tid] == tid) link[tid] = link[link[tid]];
-      wave_sync();
+    if (me) {
+      int x = root[tid];
+      while (link[x] != x) x = link[x];
+      root[tid] = x;
     }
-    if (me && root[tid] == tid) partner[tid] = tid;
-    wave_sync();
-    if (me && root[tid] == tid) atomicMax(&partner[link[tid]], tid);
-    wave_sync();
-    const int nr = me ? partner[link[root[tid]]] : 0;
-    wave_sync();
-    if (me) root[tid] = nr;
     wave_sync();
   }
   if (!valid) return;
@@ -689,7 +672,7 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int s
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
   float* scost = ecost + cap;  // scap floats
-  tree_commit<false>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, edges, ecost, wcnt,
+  tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, edges, ecost, wcnt,
                      s_sh, scost, scap);
 }
 
