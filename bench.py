@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
     ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
+    ap.add_argument("--autotune", type=int, default=1, help="pick schedule parameters by timing (0: defaults)")
+    ap.add_argument("--tuning", default="", help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave")
     ap.add_argument("--train-seeds", type=int, default=20)
     ap.add_argument("--train-per-batch", type=int, default=1500)
     ap.add_argument("--train-epochs", type=int, default=4)
@@ -169,6 +171,15 @@ def main():
     r.set_nets(pfx, pins)
     r.set_schedule(ofst, ids)
     n_nets = len(pfx) - 1
+    tuning = None
+    if args.tuning:
+        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave"]
+        r.set_tuning(**dict(zip(keys, (int(v) for v in args.tuning.split(",")))))
+        tuning = r.tuning().as_dict()
+    elif args.autotune:
+        t, tms = r.autotune(reps=2)
+        tuning = dict(t.as_dict(), autotune_ms_per_pass=round(tms, 3))
+        print(f"# autotuned {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         r.reset_demand()
         r.route_schedule()
@@ -244,7 +255,7 @@ def main():
                                f"(seed {args.seed}), {n_nets} routed after pin dedupe, "
                                f"{len(ofst) - 1} batches from 4 x-buckets",
                    "grid": [X, Y, Z], "nets": n_nets, "pairs": int(P.sum()), "batches": len(ofst) - 1,
-                   "mode": "plain"},
+                   "mode": "plain", "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kstats,

@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "load_library", "RouterError", "Router", "Model", "route_cost", "gen_grid", "gen_nets", "partition_nets",
+    "load_library", "RouterError", "Router", "Model", "Tuning", "route_cost", "gen_grid", "gen_nets", "partition_nets",
     "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
     "WORKLOADS", "device_count",
 ]
@@ -37,6 +37,15 @@ WORKLOADS = {
 
 class RouterError(RuntimeError):
     """A C-ABI call returned an error code."""
+
+
+class Tuning(ctypes.Structure):
+    """Schedule parameters (speed only, never results)."""
+    _fields_ = [("large_threads", ctypes.c_int32), ("group_max", ctypes.c_int32), ("fork", ctypes.c_int32),
+                ("refresh_blocks", ctypes.c_int32), ("lpat_wave", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 class _Desc(ctypes.Structure):
@@ -91,6 +100,9 @@ _SIGS = {
     "gr_mlp_forward": (ctypes.c_int, [_P, _P, _I32, _P]),
     "gr_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _I32, _P, ctypes.POINTER(ctypes.c_float)]),
     "gr_set_proposal": (ctypes.c_int, [_P, _P, _I32, ctypes.c_float]),
+    "gr_set_tuning": (ctypes.c_int, [_P, _P]),
+    "gr_get_tuning": (ctypes.c_int, [_P, _P]),
+    "gr_autotune": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gr_proposal_stats": (ctypes.c_int, [_P, _P, _P]),
 }
 
@@ -265,6 +277,24 @@ class Router:
         out = np.empty(self.lib.gr_lut_size(), dtype=np.float32)
         self._c(self.lib.gr_get_lut(self.h, _ptr(out)))
         return out
+
+    def tuning(self):
+        t = Tuning()
+        self._c(self.lib.gr_get_tuning(self.h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._c(self.lib.gr_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, reps=2):
+        """Time every candidate tuning on the resident schedule; keep the fastest."""
+        t = Tuning()
+        ms = ctypes.c_double()
+        self._c(self.lib.gr_autotune(self.h, reps, ctypes.byref(t), ctypes.byref(ms)))
+        return t, ms.value
 
     def set_proposal(self, model, k=8, threshold=0.6):
         """Cost only the model's top-k patterns (None: exhaustive costing)."""

@@ -20,7 +20,8 @@ using grd::kLut;
 constexpr int kNumCls = 6;
 constexpr int kClsLarge = 5;
 constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, 1 << 30};
-inline int net_class(int64_t d) {
+inline int net_class(int64_t d, int group_max) {
+  if (d > group_max) return kClsLarge;
   int k = 0;
   while (d > kClsMax[k]) ++k;
   return k;
@@ -49,6 +50,7 @@ struct Sched {
   int32_t* d_lpat = nullptr;
   float* d_cost = nullptr;
   size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0;
+  std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;
 };
@@ -70,6 +72,7 @@ struct gr_ctx {
   float *d_pfx = nullptr, *d_lut = nullptr;
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
   size_t lds_max = 65536;
+  gr_tuning tune = {1024, 64, 1, 2048, 0};
   int32_t n_nets = 0;
   std::vector<int64_t> h_pin_pfx;
   int64_t* d_pin_pfx = nullptr;

@@ -72,7 +72,7 @@ struct KTimer {
 };
 
 size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
-int refresh_grid(const gr_ctx* c) { return std::min(c->g.Z * c->M, 2048); }
+int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
 
 int refresh_all(gr_ctx* c) {
   {
@@ -118,7 +118,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       for (int i = 0; i < m.n; ++i) {
         const int32_t id = net_ids[m.begin + i];
         const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
-        const int k = net_class(d);
+        const int k = net_class(d, c->tune.group_max);
         if (k != cls) continue;
         list[lo++] = i;
         if (k == kClsLarge) {
@@ -202,7 +202,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     if (rc) return rc;
   } else if (m.P > 0) {
     KTimer t(c, GR_K_LPAT);
-    LpatFn fn = lane_kernel(c->g.Z);
+    LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
     if (fn) {
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + 255) / 256)), dim3(256), 0, c->stream, c->g, c->d_pfx,
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
@@ -250,7 +250,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     C.block0[5] = blocks;
     // small and large nets are independent: with both present, the large-net
     // workgroups run on a second stream (a parallel branch of the graph)
-    const bool fork = blocks && m.cls_n[kClsLarge] && !c->timing;
+    const bool fork = c->tune.fork && blocks && m.cls_n[kClsLarge] && !c->timing;
     hipStream_t lst = c->stream;
     if (fork) {
       HIPC(c, hipEventRecord(c->ev_fork, c->stream));
@@ -265,7 +265,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 32) / 4;
       const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
       const size_t lds = (size_t)cap * 32 + (size_t)scap * 4;
-      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(1024), lds, lst, A, cap, scap);
+      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(c->tune.large_threads), lds, lst, A, cap, scap);
     }
     if (fork) {
       HIPC(c, hipEventRecord(c->ev_join, c->stream2));
@@ -442,7 +442,11 @@ int gr_set_schedule(gr_ctx* c, int32_t n_batches, const int32_t* batch_ofst, con
   if (!c->d_pins) return fail(c, GR_ESTATE, "gr_set_nets must come first");
   if (batch_ofst[0] != 0) return fail(c, GR_EINVAL, "batch_ofst[0] must be 0");
   (void)hipSetDevice(c->dev);
-  return build_sched(c, c->sched, n_batches, batch_ofst, net_ids);
+  const int rc = build_sched(c, c->sched, n_batches, batch_ofst, net_ids);
+  if (rc) return rc;
+  c->sched.h_ofst.assign(batch_ofst, batch_ofst + n_batches + 1);
+  c->sched.h_ids.assign(net_ids, net_ids + batch_ofst[n_batches]);
+  return 0;
 }
 
 int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
@@ -540,6 +544,83 @@ int gr_get_pair_costs(gr_ctx* c, float* cost, int16_t* pat) {
 int gr_get_lut(const gr_ctx* c, float* out) {
   if (!c || !out) return GR_EINVAL;
   std::memcpy(out, c->lut, sizeof(float) * kLut);
+  return 0;
+}
+
+int gr_get_tuning(const gr_ctx* c, gr_tuning* t) {
+  if (!c || !t) return GR_EINVAL;
+  *t = c->tune;
+  return 0;
+}
+
+int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
+  if (!c || !t) return fail(c, GR_EINVAL, "null argument");
+  if ((t->large_threads != 256 && t->large_threads != 512 && t->large_threads != 1024) ||
+      (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || (t->lpat_wave != 0 && t->lpat_wave != 1))
+    return fail(c, GR_EINVAL, "tuning out of range");
+  (void)hipSetDevice(c->dev);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  const bool reclass = t->group_max != c->tune.group_max;
+  c->tune = *t;
+  c->tune.fork = t->fork ? 1 : 0;
+  Sched& s = c->sched;
+  if (s.exec) { (void)hipGraphExecDestroy(s.exec); s.exec = nullptr; }
+  s.graph_failed = false;
+  if (reclass && !s.meta.empty()) {
+    const std::vector<int32_t> ofst = s.h_ofst, ids = s.h_ids;
+    return build_sched(c, s, (int32_t)ofst.size() - 1, ofst.data(), ids.data());
+  }
+  return 0;
+}
+
+int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
+  if (!c || reps < 1) return fail(c, GR_EINVAL, "bad autotune request");
+  if (c->sched.meta.empty()) return fail(c, GR_ESTATE, "gr_set_schedule must come first");
+  (void)hipSetDevice(c->dev);
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t e0, e1;
+  HIPC(c, hipEventCreate(&e0));
+  HIPC(c, hipEventCreate(&e1));
+  const gr_tuning start = c->tune;
+  gr_tuning win = start;
+  double win_ms = 1e30;
+  const int nb = (int)c->sched.meta.size();
+  int rc = 0;
+  for (int gm : {64, 32})
+    for (int lt : {1024, 512, 256})
+      for (int fk : {1, 0})
+        for (int rb : {1024, 2048, 4096})
+          for (int lw : {0, 1}) {
+            if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z
+            gr_tuning t = {lt, gm, fk, rb, lw};
+            if ((rc = gr_set_tuning(c, &t))) goto done;
+            // one untimed pass captures the graph for this candidate
+            if ((rc = gr_reset_demand(c)) || (rc = gr_route_schedule(c, 0, nb))) goto done;
+            float ms = 0.f;
+            double tot = 0.0;
+            for (int r = 0; r < reps; ++r) {
+              if ((rc = gr_reset_demand(c))) goto done;
+              if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, GR_ENODEV, "event"); goto done; }
+              if ((rc = gr_route_schedule(c, 0, nb))) goto done;
+              if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(c, GR_ENODEV, "event timing failed");
+                goto done;
+              }
+              tot += ms;
+            }
+            if (tot / reps < win_ms) { win_ms = tot / reps; win = t; }
+          }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  if (rc) { gr_set_tuning(c, &start); return rc; }
+  if ((rc = gr_set_tuning(c, &win)) || (rc = gr_reset_demand(c))) return rc;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
   return 0;
 }
 

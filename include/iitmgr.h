@@ -157,6 +157,28 @@ int gr_lut_rmin(void);
 int gr_lut_size(void);
 
 /*
+ * Schedule parameters.  They change speed, never results (every choice is
+ * bit-identical).  Defaults: {1024, 64, 1, 2048, 0}.
+ *   large_threads   workgroup size for nets above group_max pins (256, 512 or 1024)
+ *   group_max       largest pin count handled by a wavefront group of <= 64 lanes (32 or 64)
+ *   fork            1: large-net trees on a second stream, parallel to the small ones
+ *   refresh_blocks  persistent grid of the cost refresh (>= 1)
+ *   lpat_wave       1: one wavefront per pin pair instead of one lane per pair
+ */
+typedef struct {
+  int32_t large_threads, group_max, fork, refresh_blocks, lpat_wave;
+} gr_tuning;
+int gr_set_tuning(gr_ctx* ctx, const gr_tuning* t);
+int gr_get_tuning(const gr_ctx* ctx, gr_tuning* t);
+/*
+ * Autotuner: time every candidate tuning on the resident schedule (reps
+ * passes each, demand reset before each pass; HIP events on the library
+ * stream), keep the fastest and return it with its ms per pass.  Demand is
+ * reset afterwards.  No counterpart in the reference.
+ */
+int gr_autotune(gr_ctx* ctx, int32_t reps, gr_tuning* best, double* best_ms);
+
+/*
  * Kernel timing: when enabled, every launch of the listed kernels is bracketed
  * by HIP events on the launch stream.  gr_kernel_stats returns, per kernel
  * id, the launch count and total milliseconds since the last enable.

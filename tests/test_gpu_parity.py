@@ -96,3 +96,31 @@ def test_schedule_equals_route_batch(gr, orc, gpu):
         assert _same_f32(c, np.concatenate([x[2] for x in outs]))
         assert np.array_equal(b_.demand(), a.demand())
         b_.reset_demand()
+
+
+def test_tunings_are_bit_identical(gr, gpu):
+    """Every schedule parameter choice routes identically; the autotuner keeps one of them."""
+    X, Y, Z = 256, 256, 10
+    cap = gr.gen_grid(2, 4, X, Y, Z)
+    pfx, pins = gr.gen_nets(2, 4, X, Y, Z, 5000)
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    ref = None
+    for t in ({}, dict(large_threads=256, group_max=32), dict(large_threads=512, fork=0, refresh_blocks=333),
+              dict(lpat_wave=1, group_max=64, large_threads=1024, fork=1)):
+        r.set_tuning(**t)
+        r.reset_demand()
+        r.route_schedule()
+        out = [np.asarray(a).view(np.int32) for a in r.schedule_outputs()] + [r.demand()]
+        if ref is None:
+            ref = out
+        for a, b in zip(out, ref):
+            assert np.array_equal(a, b), t
+    best, ms = r.autotune(reps=1)
+    assert ms > 0 and best.large_threads in (256, 512, 1024)
+    r.route_schedule()
+    out = [np.asarray(a).view(np.int32) for a in r.schedule_outputs()] + [r.demand()]
+    for a, b in zip(out, ref):
+        assert np.array_equal(a, b)
