@@ -68,6 +68,8 @@ _SIGS = {
     "gr_lut_size": (ctypes.c_int, []),
     "gr_last_error": (ctypes.c_char_p, [_P]),
     "gr_create": (ctypes.c_int, [ctypes.POINTER(_Desc), _P, ctypes.POINTER(_P)]),
+    "gr_create_f32": (ctypes.c_int, [ctypes.POINTER(_Desc), _P, ctypes.POINTER(_P)]),
+    "gr_get_remaining_f32": (ctypes.c_int, [_P, _P]),
     "gr_destroy": (None, [_P]),
     "gr_set_nets": (ctypes.c_int, [_P, _I32, _P, _P]),
     "gr_route_batch": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
@@ -191,14 +193,20 @@ class Router:
     """
 
     def __init__(self, X, Y, Z, cap, cost_model=COST_REF_BIT, commit_mode=COMMIT_REF_QUIRK, device=0):
+        """cap: capacity cube in WIRE_ENC order; an integer array gives the
+        integer-capacity grid (gr_create), a float array the fp32 one
+        (gr_create_f32, capacities as read from a .cap file)."""
         self.lib = load_library()
         self.X, self.Y, self.Z = X, Y, Z
-        cap = np.ascontiguousarray(cap, dtype=np.int32)
+        cap = np.asarray(cap)
+        self.f32 = cap.dtype.kind == "f"
+        cap = np.ascontiguousarray(cap, dtype=np.float32 if self.f32 else np.int32)
         if cap.size != X * Y * Z:
             raise ValueError("capacity cube has the wrong size")
         d = _Desc(X, Y, Z, cost_model, commit_mode, device)
         h = ctypes.c_void_p()
-        rc = self.lib.gr_create(ctypes.byref(d), _ptr(cap), ctypes.byref(h))
+        create = self.lib.gr_create_f32 if self.f32 else self.lib.gr_create
+        rc = create(ctypes.byref(d), _ptr(cap), ctypes.byref(h))
         if rc < 0:
             raise RouterError(f"gr_create error {rc}: {self.lib.gr_last_error(None).decode()}")
         self.h = h
@@ -259,6 +267,12 @@ class Router:
     def demand(self):
         out = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
         self._c(self.lib.gr_get_demand(self.h, _ptr(out)))
+        return out
+
+    def remaining_f32(self):
+        """Remaining fp32 capacity per cell (fp32 grids only)."""
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        self._c(self.lib.gr_get_remaining_f32(self.h, _ptr(out)))
         return out
 
     def prefix(self):

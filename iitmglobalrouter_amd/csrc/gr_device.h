@@ -26,6 +26,58 @@ __device__ __forceinline__ void dec(const Geo& g, uint32_t v, int& z, int& x, in
   else       { y = (int)(r / g.X); x = (int)(r - (uint32_t)y * g.X); }
 }
 
+// e^x in double for cell costs of grids with fp32 capacities.  Cody-Waite
+// reduction x = k ln2 + t, |t| <= ln2/2 (W. J. Cody, W. Waite, Software Manual
+// for the Elementary Functions, 1980; ln2 split into its nearest double and the
+// remainder), then the degree-13 Taylor polynomial of e^t in Horner form with
+// fused multiply-adds (truncation < 1e-17 relative), scaled by 2^k.  Every
+// step is one IEEE double operation, so host and device give the same bits.
+// Arguments above 100 give +inf and below -110 give 0 (both outside fp32's
+// finite non-zero range after rounding).
+__host__ __device__ inline double cw_exp(double x) {
+  if (x > 100.0) return __builtin_inf();
+  if (x < -110.0) return 0.0;
+  const double kInvLn2 = 1.4426950408889634;
+  const double kLn2Hi = 0.6931471805599453;         // nearest double to ln 2
+  const double kLn2Lo = 2.3190468138462996e-17;     // ln 2 - kLn2Hi
+  const double k = __builtin_rint(x * kInvLn2);
+  double t = __builtin_fma(-k, kLn2Hi, x);
+  t = __builtin_fma(-k, kLn2Lo, t);
+  double p = 1.0 / 6227020800.0;                    // 1/13!
+  p = __builtin_fma(p, t, 1.0 / 479001600.0);
+  p = __builtin_fma(p, t, 1.0 / 39916800.0);
+  p = __builtin_fma(p, t, 1.0 / 3628800.0);
+  p = __builtin_fma(p, t, 1.0 / 362880.0);
+  p = __builtin_fma(p, t, 1.0 / 40320.0);
+  p = __builtin_fma(p, t, 1.0 / 5040.0);
+  p = __builtin_fma(p, t, 1.0 / 720.0);
+  p = __builtin_fma(p, t, 1.0 / 120.0);
+  p = __builtin_fma(p, t, 1.0 / 24.0);
+  p = __builtin_fma(p, t, 1.0 / 6.0);
+  p = __builtin_fma(p, t, 0.5);
+  p = __builtin_fma(p, t, 1.0);
+  p = __builtin_fma(p, t, 1.0);
+  return __builtin_ldexp(p, (int)k);
+}
+
+// Cost of a cell with fp32 remaining capacity r under cost model `model`
+// (0: exp(-r/2), bit_util.hpp:13-15; 1: f(r - 1) - f(r) with r - 1 in fp32,
+// util.hpp:127-129; 2: exp(1 - r), README.md:10), saturated at 1e30 like the
+// integer-capacity table.  For integral r it equals that table's entry.
+__host__ __device__ inline float cost_f32(int model, float r) {
+  float v;
+  if (model == 1) {
+    const float r1 = r - 1.0f;
+    v = (float)cw_exp(-0.5 * (double)r1) - (float)cw_exp(-0.5 * (double)r);
+  } else if (model == 2) {
+    v = (float)cw_exp(1.0 - (double)r);
+  } else {
+    v = (float)cw_exp(-0.5 * (double)r);
+  }
+  if (!(v < 1e30f)) v = 1e30f;
+  return v;
+}
+
 // Orderable key of an fp32 cost: -0 and +0 map to the same key, negatives
 // below positives, so (key << 32 | index) orders lexicographically by
 // (cost, index).

@@ -71,6 +71,7 @@ struct gr_ctx {
   int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;
   float *d_pfx = nullptr, *d_lut = nullptr;
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
+  float *d_remf = nullptr, *d_capf0 = nullptr;  // fp32 grids: remaining and initial capacity
   size_t lds_max = 65536;
   gr_tuning tune = {1024, 64, 1, 2048, 0};
   int32_t n_nets = 0;

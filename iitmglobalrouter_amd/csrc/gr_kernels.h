@@ -2,7 +2,9 @@
 //
 // Device state (WIRE_ENC cell order, util.hpp:38-44; every layer is a set of
 // contiguous runs: rows of length X on even layers, columns of length Y on odd):
-//   rem[cell]        int32  remaining capacity cap - demand (the integer state)
+//   rem[cell]        int32  remaining capacity cap - demand (the integer state);
+//                           with fp32 capacities: minus the demand, and
+//   remf[cell]       fp32   remaining capacity r (fp32 grids only)
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
@@ -31,11 +33,17 @@ namespace grd {
 // [i - 2^k, i)) is the pairwise fp32 sum of its aligned block, built level by
 // level in LDS, and Q(x) accumulates nodes in the reference's descent order
 // (bit_util.hpp:19-35).
+//
+// FLT: grids with fp32 capacities.  remf holds r per cell; a stamped cell's r
+// is lowered by 1.0f in fp32 (bit_util.hpp:87, d_cube_data_bit[tid]--) and the
+// integer rem counts the commits; the cell cost is cost_f32(model, r).
+template <bool FLT>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  const int32_t* __restrict__ stamp, float* __restrict__ pfx,
                                                  const float* __restrict__ lut,
                                                  const int32_t* __restrict__ run_stamp,
-                                                 const int32_t* __restrict__ ctr, int all) {
+                                                 const int32_t* __restrict__ ctr, int all,
+                                                 float* __restrict__ remf, int model) {
   extern __shared__ float sh[];
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
@@ -52,13 +60,23 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     const int L = (z & 1) ? g.Y : g.X;
     const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
-      int v = rem[base + i];
-      if (!all && stamp[base + i] == bstamp) {
-        v -= 1;
-        rem[base + i] = v;
+      if constexpr (FLT) {
+        float r = remf[base + i];
+        if (!all && stamp[base + i] == bstamp) {
+          r -= 1.0f;
+          remf[base + i] = r;
+          rem[base + i] -= 1;
+        }
+        lv[i] = cost_f32(model, r);
+      } else {
+        int v = rem[base + i];
+        if (!all && stamp[base + i] == bstamp) {
+          v -= 1;
+          rem[base + i] = v;
+        }
+        v = v < kRMin ? kRMin : (v > kRMin + kLut - 1 ? kRMin + kLut - 1 : v);
+        lv[i] = slut[v - kRMin];
       }
-      v = v < kRMin ? kRMin : (v > kRMin + kLut - 1 ? kRMin + kLut - 1 : v);
-      lv[i] = slut[v - kRMin];
     }
     if (threadIdx.x == 0) {
       int off = 0;

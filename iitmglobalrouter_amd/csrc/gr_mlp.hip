@@ -482,6 +482,7 @@ extern "C" {
 int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples, gr_mlp** out) {
   if (!c || !out || hidden < 32 || hidden % 32 || max_samples < 0) return fail(c, GR_EINVAL, "bad model shape");
   *out = nullptr;
+  if (c->d_remf) return fail(c, GR_ESTATE, "the learned proposal needs a grid with integer capacities");
   (void)hipSetDevice(c->dev);
   gr_mlp* m = new gr_mlp();
   m->ctx = c;
@@ -577,7 +578,8 @@ int gr_mlp_set_params(gr_mlp* m, const float* in) {
 }
 
 int gr_mlp_collect(gr_ctx* c, gr_mlp* m, int32_t per_batch) {
-  if (!c || (per_batch > 0 && !m) || per_batch < 0) return fail(c, GR_EINVAL, "bad collect request");
+  if (!c || (per_batch > 0 && (!m || m->ctx != c)) || per_batch < 0)
+    return fail(c, GR_EINVAL, "bad collect request (the model must belong to this context)");
   if (per_batch > 0 && c->prop) return fail(c, GR_ESTATE, "collect labels from exhaustive costing: clear the proposal first");
   c->collect = per_batch > 0 ? m : nullptr;
   c->collect_per_batch = per_batch;
@@ -629,7 +631,7 @@ int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t se
 }
 
 int gr_mlp_features(gr_ctx* c, gr_mlp* m, const int32_t* src, const int32_t* dst, int32_t n, float* out) {
-  if (!c || !m || n < 0 || (n && (!src || !dst || !out))) return fail(c, GR_EINVAL, "bad feature request");
+  if (!c || !m || m->ctx != c || n < 0 || (n && (!src || !dst || !out))) return fail(c, GR_EINVAL, "bad feature request");
   if (!n) return 0;
   (void)hipSetDevice(c->dev);
   int32_t *ds = nullptr, *dd = nullptr;

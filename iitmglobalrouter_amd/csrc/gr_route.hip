@@ -74,11 +74,19 @@ struct KTimer {
 size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
 int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
 
+void launch_refresh(gr_ctx* c, int all) {
+  if (c->d_remf)
+    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
+                       c->d_rem, c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+  else
+    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
+                       c->d_rem, c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+}
+
 int refresh_all(gr_ctx* c) {
   {
     KTimer t(c, GR_K_REFRESH);
-    hipLaunchKernelGGL(k_refresh, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M, c->d_rem,
-                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, 1);
+    launch_refresh(c, 1);
   }
   HIPC(c, hipGetLastError());
   return 0;
@@ -274,8 +282,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   }
   {
     KTimer t(c, GR_K_REFRESH);
-    hipLaunchKernelGGL(k_refresh, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M, c->d_rem,
-                       c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, 0);
+    launch_refresh(c, 0);
   }
   HIPC(c, hipGetLastError());
   c->last_P = m.P;
@@ -311,8 +318,11 @@ int gr_lut_size(void) { return kLut; }
 
 const char* gr_last_error(const gr_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
-int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
-  if (!d || !cap || !out) return fail(nullptr, GR_EINVAL, "null argument");
+namespace {
+
+// Shared set-up of gr_create / gr_create_f32: exactly one of cap, capf is set.
+int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_ctx** out) {
+  if (!d || (!cap && !capf) || !out) return fail(nullptr, GR_EINVAL, "null argument");
   *out = nullptr;
   if (d->X < 1 || d->Y < 1 || d->X > 8192 || d->Y > 8192 || d->Z < 3 || d->Z > 32)
     return fail(nullptr, GR_ERANGE, "grid must have 1 <= X, Y <= 8192 and 3 <= Z <= 32");
@@ -346,18 +356,31 @@ int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_stamp, cs)) ||
       (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
-      (rc = dalloc(c, &c->d_ctr, 1))) {
+      (rc = dalloc(c, &c->d_ctr, 1)) ||
+      (capf && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs))))) {
     g_err = c->err;
     return bail(rc);
   }
   c->lds_max = prop.sharedMemPerBlock;
-  if (hipFuncSetAttribute((const void*)k_refresh, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)refresh_lds(c)) != hipSuccess ||
+  bool ok = hipFuncSetAttribute((const void*)k_refresh<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)refresh_lds(c)) == hipSuccess &&
+            hipFuncSetAttribute((const void*)k_refresh<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)refresh_lds(c)) == hipSuccess;
+  if (cap) {
+    ok = ok && hipMemcpy(c->d_cap0, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_rem, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) == hipSuccess;
+  } else {
+    // fp32 grid: the integer state starts at 0 and counts commits downwards,
+    // so cap0 - rem is still the demand
+    ok = ok && hipMemset(c->d_cap0, 0, sizeof(int32_t) * cs) == hipSuccess &&
+         hipMemset(c->d_rem, 0, sizeof(int32_t) * cs) == hipSuccess &&
+         hipMemcpy(c->d_capf0, capf, sizeof(float) * cs, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_remf, capf, sizeof(float) * cs, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok ||
       hipFuncSetAttribute((const void*)k_tree_large, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
-      hipMemcpy(c->d_cap0, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_rem, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_stamp, 0, sizeof(int32_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
@@ -372,12 +395,25 @@ int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
   return 0;
 }
 
+}  // namespace
+
+int gr_create(const gr_grid_desc* d, const int32_t* cap, gr_ctx** out) {
+  if (!cap) return fail(nullptr, GR_EINVAL, "null argument");
+  return create_ctx(d, cap, nullptr, out);
+}
+
+int gr_create_f32(const gr_grid_desc* d, const float* cap, gr_ctx** out) {
+  if (!cap) return fail(nullptr, GR_EINVAL, "null argument");
+  return create_ctx(d, nullptr, cap, out);
+}
+
 void gr_destroy(gr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_stamp); (void)hipFree(c->d_run_stamp);
   (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
+  (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
   (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat);
   free_sched(c->sched);
@@ -510,7 +546,18 @@ int gr_reset_demand(gr_ctx* c) {
   if (!c) return fail(c, GR_EINVAL, "null context");
   (void)hipSetDevice(c->dev);
   HIPC(c, hipMemcpyAsync(c->d_rem, c->d_cap0, sizeof(int32_t) * c->cs, hipMemcpyDeviceToDevice, c->stream));
+  if (c->d_remf)
+    HIPC(c, hipMemcpyAsync(c->d_remf, c->d_capf0, sizeof(float) * c->cs, hipMemcpyDeviceToDevice, c->stream));
   return refresh_all(c);
+}
+
+int gr_get_remaining_f32(gr_ctx* c, float* out) {
+  if (!c || !out) return fail(c, GR_EINVAL, "null argument");
+  if (!c->d_remf) return fail(c, GR_ESTATE, "the grid has integer capacities (gr_get_demand)");
+  (void)hipSetDevice(c->dev);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(out, c->d_remf, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int gr_get_demand(gr_ctx* c, int32_t* out) {

@@ -79,6 +79,21 @@ int gr_device_count(void);
  */
 int gr_create(const gr_grid_desc* desc, const int32_t* cap, gr_ctx** out);
 
+/*
+ * Same for a grid with fp32 capacities (the .cap file's values,
+ * read_data.hpp:136-143).  The remaining capacity r is kept per cell in fp32
+ * and lowered by 1.0f when a batch commits the cell (bit_util.hpp:87); a cell
+ * costs f(r) evaluated in double by a fixed Cody-Waite/Taylor sequence and
+ * rounded to fp32 (equal to the integer table for integral r).  gr_get_demand
+ * still returns the integer demand.  The learned proposal needs integer
+ * capacities and refuses such a context.
+ * Replaces: the float cube upload and init_bit_data as for gr_create.
+ */
+int gr_create_f32(const gr_grid_desc* desc, const float* cap, gr_ctx** out);
+
+/* Remaining fp32 capacity per cell of a gr_create_f32 context (GR_ESTATE otherwise). */
+int gr_get_remaining_f32(gr_ctx* ctx, float* out);
+
 /* Free every device and host resource of the context. */
 void gr_destroy(gr_ctx* ctx);
 

@@ -22,6 +22,11 @@ _SIGS = {
     "orc_lut_rmin": (ctypes.c_int, []),
     "orc_lut_size": (ctypes.c_int, []),
     "orc_create": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    "orc_create_f32": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    "orc_get_remf": (ctypes.c_int, [_P, _P]),
+    "orc_set_remf": (ctypes.c_int, [_P, _P, _P]),
+    "orc_cw_exp": (ctypes.c_double, [ctypes.c_double]),
+    "orc_cost_f32": (ctypes.c_float, [ctypes.c_int, ctypes.c_float]),
     "orc_destroy": (None, [_P]),
     "orc_set_nets": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "orc_get_demand": (ctypes.c_int, [_P, _P]),
@@ -80,11 +85,29 @@ def boruvka(n, costs):
     return edges, nc.value, rounds
 
 
+def cw_exp(x):
+    """The oracle's e^x in double (Cody-Waite reduction + Taylor polynomial)."""
+    return lib().orc_cw_exp(float(x))
+
+
+def cost_f32(model, r):
+    """Cell cost of fp32 remaining capacity r under a cost model."""
+    return np.float32(lib().orc_cost_f32(int(model), float(np.float32(r))))
+
+
 class Oracle:
     def __init__(self, X, Y, Z, cap, cost_model=0, commit_mode=0):
+        """An integer capacity array gives the integer-capacity grid, a float
+        array the fp32 one."""
         self.X, self.Y, self.Z = X, Y, Z
-        cap = np.ascontiguousarray(cap, dtype=np.int32)
-        self.h = lib().orc_create(X, Y, Z, cost_model, commit_mode, _ptr(cap))
+        cap = np.asarray(cap)
+        self.f32 = cap.dtype.kind == "f"
+        if self.f32:
+            cap = np.ascontiguousarray(cap, dtype=np.float32)
+            self.h = lib().orc_create_f32(X, Y, Z, cost_model, commit_mode, _ptr(cap))
+        else:
+            cap = np.ascontiguousarray(cap, dtype=np.int32)
+            self.h = lib().orc_create(X, Y, Z, cost_model, commit_mode, _ptr(cap))
         if not self.h:
             raise ValueError("bad oracle grid")
 
@@ -106,6 +129,16 @@ class Oracle:
     def set_rem(self, rem):
         r = np.ascontiguousarray(rem, dtype=np.int32)
         lib().orc_set_rem(self.h, _ptr(r))
+
+    def remaining_f32(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        assert lib().orc_get_remf(self.h, _ptr(out)) == 0
+        return out
+
+    def set_remaining_f32(self, remf, demand):
+        r = np.ascontiguousarray(remf, dtype=np.float32)
+        d = np.ascontiguousarray(demand, dtype=np.int32)
+        assert lib().orc_set_remf(self.h, _ptr(r), _ptr(d)) == 0
 
     def route_batch(self, net_ids, nthreads=0):
         ids = np.ascontiguousarray(net_ids, dtype=np.int32)

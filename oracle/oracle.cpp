@@ -21,7 +21,8 @@
 // Canonical arithmetic (the reference is nondeterministic here; these choices
 // make every output reproducible and are shared with the HIP path):
 //   * cell cost c = LUT[clamp(cap - dem, -256, 255)], LUT entry = fp32 rounding
-//     of the model evaluated in double, saturated at 1e30;
+//     of the model evaluated in double, saturated at 1e30; with fp32
+//     capacities c = cost_f32(model, r) with r lowered by 1.0f per commit;
 //   * Fenwick node covering an aligned block = pairwise fp32 sum of its cells,
 //     recomputed from integer demand after every batch (the reference builds
 //     and updates nodes with float atomicAdd, bit_util.hpp:44,51);
@@ -67,6 +68,39 @@ void build_lut(int model, float* lut) {
   }
 }
 
+// Grids with fp32 capacities: e^x by Cody-Waite reduction x = k ln2 + t
+// (Cody and Waite, Software Manual for the Elementary Functions, 1980) and the
+// degree-13 Taylor polynomial of e^t in Horner form with fused multiply-adds,
+// scaled by 2^k; rounded once to fp32 by the caller.  The cost functions are
+// those of the table above with r in fp32 (util.hpp:127-129: r - 1 in fp32).
+double cw_exp(double x) {
+  if (x > 100.0) return HUGE_VAL;
+  if (x < -110.0) return 0.0;
+  const double k = std::rint(x * 1.4426950408889634);
+  double t = std::fma(-k, 0.6931471805599453, x);
+  t = std::fma(-k, 2.3190468138462996e-17, t);
+  static const double inv_fact[14] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
+                                      1.0 / 40320, 1.0 / 362880, 1.0 / 3628800, 1.0 / 39916800,
+                                      1.0 / 479001600, 1.0 / 6227020800.0};
+  double p = inv_fact[13];
+  for (int n = 12; n >= 0; --n) p = std::fma(p, t, inv_fact[n]);
+  return std::ldexp(p, (int)k);
+}
+
+float cost_f32(int model, float r) {
+  float v;
+  if (model == 1) {
+    float r1 = r - 1.0f;
+    v = (float)cw_exp(-0.5 * (double)r1) - (float)cw_exp(-0.5 * (double)r);
+  } else if (model == 2) {
+    v = (float)cw_exp(1.0 - (double)r);
+  } else {
+    v = (float)cw_exp(-0.5 * (double)r);
+  }
+  if (!(v < kSat)) v = kSat;
+  return v;
+}
+
 inline uint32_t ord_key(float f) {
   uint32_t u;
   f = f + 0.0f;  // -0 -> +0 so that -0 and +0 compare equal as in float ==
@@ -78,6 +112,8 @@ struct Ctx {
   int X, Y, Z, Z2, model, commit_mode;
   int64_t cs;
   std::vector<int32_t> cap0, rem, stamp;
+  bool fmode = false;       // fp32 capacities: remf is the state, rem counts -demand
+  std::vector<float> remf;
   std::vector<float> pfx;  // materialised Q(x) per cell, WIRE_ENC order
   float lut[kLut];
   int32_t bstamp = 0;
@@ -111,6 +147,9 @@ struct Ctx {
     int i = r < kRMin ? kRMin : (r > kRMax ? kRMax : r);
     return lut[i - kRMin];
   }
+  inline float cell_cost(int64_t cell) const {
+    return fmode ? cost_f32(model, remf[cell]) : cost_of_rem(rem[cell]);
+  }
 
   // Rebuild the materialised prefix of one run (z, line) from integer state.
   void refresh_run(int z, int line) {
@@ -118,7 +157,7 @@ struct Ctx {
     int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
     std::vector<std::vector<float>> lv;
     lv.emplace_back(L);
-    for (int i = 0; i < L; ++i) lv[0][i] = cost_of_rem(rem[base + i]);
+    for (int i = 0; i < L; ++i) lv[0][i] = cell_cost(base + i);
     for (int k = 1; (L >> k) > 0; ++k) {
       int n = L >> k;
       lv.emplace_back(n);
@@ -324,6 +363,37 @@ void* orc_create(int X, int Y, int Z, int cost_model, int commit_mode, const int
   return c;
 }
 
+// fp32 capacities (read_data.hpp:136-143): the integer state starts at 0 and
+// counts commits downwards, so cap0 - rem is still the demand.
+void* orc_create_f32(int X, int Y, int Z, int cost_model, int commit_mode, const float* cap) {
+  if (X <= 0 || Y <= 0 || Z < 2 || !cap) return nullptr;
+  std::vector<int32_t> zero((size_t)X * Y * Z, 0);
+  Ctx* c = (Ctx*)orc_create(X, Y, Z, cost_model, commit_mode, zero.data());
+  c->fmode = true;
+  c->remf.assign(cap, cap + c->cs);
+  c->refresh_all();
+  return c;
+}
+
+int orc_get_remf(void* h, float* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c->fmode) return -1;
+  std::memcpy(out, c->remf.data(), sizeof(float) * c->cs);
+  return 0;
+}
+
+int orc_set_remf(void* h, const float* remf, const int32_t* demand) {
+  Ctx* c = (Ctx*)h;
+  if (!c->fmode) return -1;
+  c->remf.assign(remf, remf + c->cs);
+  for (int64_t i = 0; i < c->cs; ++i) c->rem[i] = -demand[i];
+  c->refresh_all();
+  return 0;
+}
+
+double orc_cw_exp(double x) { return cw_exp(x); }
+float orc_cost_f32(int model, float r) { return cost_f32(model, r); }
+
 void orc_destroy(void* h) { delete (Ctx*)h; }
 
 int orc_set_nets(void* h, int n_nets, const int64_t* pin_pfx, const int32_t* pins) {
@@ -373,7 +443,7 @@ double orc_pattern_cost_walk(void* h, int32_t src, int32_t dst, int p, int* vali
   bool D = (x1 != x2 && y1 != y2);
   int z = z1, x = x1, y = y1;
   double s = 0.0;
-  auto cell = [&](int zz, int xx, int yy) { return (double)c->cost_of_rem(c->rem[c->enc(zz, xx, yy)]); };
+  auto cell = [&](int zz, int xx, int yy) { return (double)c->cell_cost(c->enc(zz, xx, yy)); };
   auto via = [&](int za, int zb) {
     for (int l = std::min(za, zb); l < std::max(za, zb); ++l) s += cell(l, x, y);
   };
@@ -479,6 +549,7 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
       if (c->stamp[cell] == c->bstamp) continue;
       c->stamp[cell] = c->bstamp;
       c->rem[cell] -= 1;
+      if (c->fmode) c->remf[cell] -= 1.0f;  // bit_util.hpp:87
       ++touched;
       int z, x, y;
       c->dec(cell, z, x, y);
