@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 __all__ = [
-    "load_library", "RouterError", "Router", "Model", "Tuning", "route_cost", "gen_grid", "gen_nets", "partition_nets",
+    "load_library", "RouterError", "Router", "Model", "Tuning", "Design", "route_cost", "gen_grid", "gen_nets",
+    "partition_nets", "FMT_FIRST_ACCESS_POINT", "FMT_NEW", "WRITE_DBU", "WRITE_MERGE", "WRITE_PATCH",
     "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
     "WORKLOADS", "device_count",
 ]
@@ -106,6 +107,20 @@ _SIGS = {
     "gr_get_tuning": (ctypes.c_int, [_P, _P]),
     "gr_autotune": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gr_proposal_stats": (ctypes.c_int, [_P, _P, _P]),
+    "gr_design_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32, _I32, ctypes.POINTER(_P)]),
+    "gr_design_create": (ctypes.c_int, [_I32, _I32, _I32, _P, _I32, _P, _P, ctypes.POINTER(_P)]),
+    "gr_design_free": (None, [_P]),
+    "gr_design_info": (ctypes.c_int, [_P, _P]),
+    "gr_design_capacity": (ctypes.c_int, [_P, _P]),
+    "gr_design_layers": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gr_design_layer_name": (ctypes.c_int, [_P, _I32, ctypes.c_char_p, _I32]),
+    "gr_design_gcell_lengths": (ctypes.c_int, [_P, _P, _P]),
+    "gr_design_nets": (ctypes.c_int, [_P, _P, _P]),
+    "gr_design_single_nets": (ctypes.c_int, [_P, _P]),
+    "gr_design_net_name": (ctypes.c_int, [_P, _I32, ctypes.c_char_p, _I32]),
+    "gr_design_save": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p]),
+    "gr_write_routes": (ctypes.c_int, [_P, ctypes.c_char_p, _I32, _I32, _P, _P, _P, _P, _I32,
+                                       ctypes.POINTER(_I64)]),
 }
 
 
@@ -183,6 +198,115 @@ def route_cost(cap, demand):
     wl = int(d.sum())
     of = int(np.maximum(d - cap, 0).sum())
     return {"wirelength": wl, "overflow": of, "total": wl + of}
+
+
+FMT_FIRST_ACCESS_POINT, FMT_NEW = 1, 2
+WRITE_DBU, WRITE_MERGE, WRITE_PATCH = 1, 2, 4
+
+
+class _DesignDesc(ctypes.Structure):
+    _fields_ = [("X", ctypes.c_int32), ("Y", ctypes.c_int32), ("Z", ctypes.c_int32), ("n_nets", ctypes.c_int32),
+                ("n_pins", ctypes.c_int64), ("n_single", ctypes.c_int32), ("unit_length_cost", ctypes.c_float),
+                ("unit_via_cost", ctypes.c_float)]
+
+
+class Design:
+    """Grid capacities and named nets of a design (host only).
+
+    Mirrors READ_DATA (read_data.hpp): ``Design.read(cap, net)`` parses an
+    ISPD .cap/.net pair; ``Design(X, Y, Z, cap, pin_pfx, pin_enc)`` builds one
+    from arrays.  Routed nets (>= 2 distinct pins) keep file order and are the
+    router's net ids; one-pin nets are kept aside for the writer.
+    """
+
+    def __init__(self, X=None, Y=None, Z=None, cap=None, pin_pfx=None, pin_enc=None, _handle=None):
+        self.lib = load_library()
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = ctypes.c_void_p()
+            pfx = np.ascontiguousarray(pin_pfx, dtype=np.int64)
+            pins = np.ascontiguousarray(pin_enc, dtype=np.int32)
+            c = None if cap is None else np.ascontiguousarray(cap, dtype=np.float32)
+            _check(self.lib.gr_design_create(X, Y, Z, _ptr(c), len(pfx) - 1, _ptr(pfx), _ptr(pins), ctypes.byref(h)))
+            self.h = h
+        info = _DesignDesc()
+        _check(self.lib.gr_design_info(self.h, ctypes.byref(info)))
+        self.X, self.Y, self.Z = info.X, info.Y, info.Z
+        self.n_nets, self.n_pins, self.n_single = info.n_nets, info.n_pins, info.n_single
+        self.unit_length_cost, self.unit_via_cost = info.unit_length_cost, info.unit_via_cost
+
+    @classmethod
+    def read(cls, cap_path, net_path, flags=0, threads=0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.gr_design_read(os.fsencode(cap_path), os.fsencode(net_path), flags, threads, ctypes.byref(h)))
+        return cls(_handle=h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gr_design_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def capacity(self):
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        _check(self.lib.gr_design_capacity(self.h, _ptr(out)))
+        return out
+
+    def layers(self):
+        d = np.empty(self.Z, np.int32)
+        m = np.empty(self.Z, np.int32)
+        c = np.empty(self.Z, np.float32)
+        _check(self.lib.gr_design_layers(self.h, _ptr(d), _ptr(m), _ptr(c)))
+        names = []
+        buf = ctypes.create_string_buffer(256)
+        for z in range(self.Z):
+            _check(self.lib.gr_design_layer_name(self.h, z, buf, 256))
+            names.append(buf.value.decode())
+        return {"name": names, "dir": d, "min_len": m, "unit_short_cost": c}
+
+    def gcell_lengths(self):
+        h = np.empty(max(self.X - 1, 1), np.int64)
+        v = np.empty(max(self.Y - 1, 1), np.int64)
+        _check(self.lib.gr_design_gcell_lengths(self.h, _ptr(h), _ptr(v)))
+        return h[: self.X - 1], v[: self.Y - 1]
+
+    def nets(self):
+        pfx = np.empty(self.n_nets + 1, np.int64)
+        pins = np.empty(max(self.n_pins, 1), np.int32)
+        _check(self.lib.gr_design_nets(self.h, _ptr(pfx), _ptr(pins)))
+        return pfx, pins[: self.n_pins]
+
+    def single_nets(self):
+        out = np.empty(max(self.n_single, 1), np.int32)
+        _check(self.lib.gr_design_single_nets(self.h, _ptr(out)))
+        return out[: self.n_single]
+
+    def net_name(self, i):
+        buf = ctypes.create_string_buffer(1024)
+        n = _check(self.lib.gr_design_net_name(self.h, i, buf, 1024))
+        if n >= 1024:
+            buf = ctypes.create_string_buffer(n + 1)
+            _check(self.lib.gr_design_net_name(self.h, i, buf, n + 1))
+        return buf.value.decode()
+
+    def save(self, cap_path=None, net_path=None):
+        _check(self.lib.gr_design_save(self.h, None if cap_path is None else os.fsencode(cap_path),
+                                       None if net_path is None else os.fsencode(net_path)))
+
+    def write_routes(self, path, batch_ofst, net_ids, edges, lpat, flags=0, threads=0):
+        """Write the routes of a schedule (gr_write_routes); returns bytes written."""
+        ofst = np.ascontiguousarray(batch_ofst, dtype=np.int32)
+        ids = np.ascontiguousarray(net_ids, dtype=np.int32)
+        e = np.ascontiguousarray(edges, dtype=np.int64)
+        l = np.ascontiguousarray(lpat, dtype=np.int32)
+        nbytes = ctypes.c_int64()
+        _check(self.lib.gr_write_routes(self.h, os.fsencode(path), flags, len(ofst) - 1, _ptr(ofst), _ptr(ids),
+                                        _ptr(e), _ptr(l), threads, ctypes.byref(nbytes)))
+        return nbytes.value
 
 
 class Router:

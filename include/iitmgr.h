@@ -46,7 +46,7 @@ extern "C" {
 #define GR_ESTATE -4    /* call out of order (e.g. routing before gr_set_nets) */
 #define GR_ERANGE -5    /* size beyond a documented limit */
 
-/* cost models: all map integer remaining capacity r = cap - demand to fp32 */
+/* cost models: all map remaining capacity r = cap - demand to fp32 */
 #define GR_COST_REF_BIT 0   /* exp(-r/2): bit_util.hpp:13-15 (paper -bit_pmodel run) */
 #define GR_COST_REF_INC 1   /* exp(-(r-1)/2) - exp(-r/2): util.hpp:127-129 */
 #define GR_COST_LIT_EXP 2   /* exp(d + 1 - c) = exp(1 - r): README.md:10 */
@@ -251,7 +251,8 @@ int gr_proposal_stats(gr_ctx* ctx, int64_t* pairs, int64_t* fallbacks);
 /* ---- host side of the path (no device work) ---- */
 
 /*
- * Synthetic workload generator (seeded, deterministic; spec in DESIGN.md).
+ * Synthetic workload generator (seeded, deterministic; the distributions of
+ * each family are fixed in gr_host.cpp).
  * kind: 1..5 selects the workload family.  First call with NULL buffers to
  * learn sizes (n_nets, n_pins); then call again with buffers.
  * cap gets X*Y*Z values; pin_pfx n_nets+1; pin_enc n_pins.
@@ -270,6 +271,82 @@ int gr_gen_nets(int32_t kind, uint64_t seed, int32_t X, int32_t Y, int32_t Z, in
 int gr_partition_nets(int32_t X, int32_t Y, int32_t Z, int32_t n_nets, const int64_t* pin_pfx,
                       const int32_t* pin_enc, int32_t num_xdim_parts,
                       int32_t* batch_ofst, int32_t* net_ids);
+
+/* ---- design database, ISPD files and route writer (host only) ---- */
+
+typedef struct gr_design gr_design;
+
+typedef struct {
+  int32_t X, Y, Z;
+  int32_t n_nets;      /* nets with >= 2 distinct pins: the routed nets, file order */
+  int64_t n_pins;      /* their pins */
+  int32_t n_single;    /* nets with one distinct pin (written, never routed) */
+  float unit_length_cost, unit_via_cost;
+} gr_design_desc;
+
+/* .net dialects (read_data.hpp:540-546): one access point per pin
+ * "[(z, x, y)]" (read_ispd24_net_single, :314-333, default); the first of
+ * several access points (read_ispd24_net, :284-313); the -new_fmt dialect
+ * (read_nt, :262-283, whose .cap header also carries four more numbers). */
+#define GR_FMT_FIRST_ACCESS_POINT 1
+#define GR_FMT_NEW 2
+
+/*
+ * Read an ISPD .cap / .net pair.  Capacities are parsed with the reference's
+ * fp32 digit accumulation (bit-identical values), pins of a net are
+ * deduplicated and sorted by cell, nets with one pin are set aside, the rest
+ * keep file order and are the router's net ids 0..n_nets-1.  threads <= 0
+ * picks min(hardware threads, 16).
+ * Replaces READ_DATA::read_cap_file and read_net_file (read_data.hpp:94-195,
+ * 196-345, 366-432, 534-565).
+ */
+int gr_design_read(const char* cap_path, const char* net_path, int32_t flags, int32_t threads, gr_design** out);
+
+/* A design from arrays (fp32 capacities, may be NULL for zeros; nets named
+ * "net<i>"); pins are deduplicated and one-pin nets set aside as when reading. */
+int gr_design_create(int32_t X, int32_t Y, int32_t Z, const float* cap, int32_t n_nets, const int64_t* pin_pfx,
+                     const int32_t* pin_enc, gr_design** out);
+void gr_design_free(gr_design* d);
+int gr_design_info(const gr_design* d, gr_design_desc* out);
+/* X*Y*Z fp32 capacities, WIRE_ENC order (read_layer_data, read_data.hpp:136-143). */
+int gr_design_capacity(const gr_design* d, float* cap);
+/* Per layer: direction flag, minimum length, unit short cost (any may be NULL). */
+int gr_design_layers(const gr_design* d, int32_t* dir, int32_t* min_len, float* unit_short_cost);
+/* Layer name into buf (NUL-terminated, truncated to size-1); returns its length. */
+int gr_design_layer_name(const gr_design* d, int32_t z, char* buf, int32_t size);
+/* GCell lengths: hlen X-1 values, vlen Y-1 values (read_data.hpp:116-131). */
+int gr_design_gcell_lengths(const gr_design* d, int64_t* hlen, int64_t* vlen);
+/* Routed nets as gr_set_nets takes them: pin_pfx n_nets+1, pin_enc n_pins. */
+int gr_design_nets(const gr_design* d, int64_t* pin_pfx, int32_t* pin_enc);
+/* The pin cell of each one-pin net (n_single values). */
+int gr_design_single_nets(const gr_design* d, int32_t* pin_enc);
+/* Name of routed net i (0..n_nets-1) or one-pin net n_nets+j, with '\n' and
+ * ' ' removed as the writer prints it; returns its length. */
+int gr_design_net_name(const gr_design* d, int32_t net, char* buf, int32_t size);
+/* Write the design back as .cap / .net text (either path may be NULL); the
+ * .net side is write_ispd24_nets' format (read_data.hpp:349-359). */
+int gr_design_save(const gr_design* d, const char* cap_path, const char* net_path);
+
+/* route text formats (writer_util.hpp; the reference's -rseg_fmt, -wrt_mrg, -oprd_ppatch) */
+#define GR_WRITE_DBU 1    /* "4200g+2100 ... metalN" lines instead of grid "z x y z x y" */
+#define GR_WRITE_MERGE 2  /* per net: fuse touching wires, distinct via cells */
+#define GR_WRITE_PATCH 4  /* with MERGE: split wires at pins that are no segment end */
+
+/*
+ * Write the routes of a schedule: for every batch in order and every net of
+ * the batch in order, the net's tree edges (gr_get_schedule_outputs layout:
+ * deg slots per net, n-1 edges and a -1 sentinel) expanded to segments along
+ * each edge's L-pattern; one-pin nets (one via each) follow the first batch.
+ * Each net is "name\n(\n" + one line per segment + ")\n".  Output is
+ * deterministic; threads <= 0 picks min(hardware threads, 16).  *bytes (may
+ * be NULL) gets the file size.
+ * Replaces ROUTE_WRITER::write_route (writer_files/writer.hpp:12-51) and
+ * ROUTE_WRITER_UTIL::write_route / mrg_route_write / write_odeg_route
+ * (writer_files/writer_util.hpp:101-165, 290-364).
+ */
+int gr_write_routes(const gr_design* d, const char* path, int32_t flags, int32_t n_batches, const int32_t* batch_ofst,
+                    const int32_t* net_ids, const int64_t* edges, const int32_t* lpat, int32_t threads,
+                    int64_t* bytes);
 
 #ifdef __cplusplus
 }
