@@ -107,6 +107,8 @@ _SIGS = {
     "gr_get_tuning": (ctypes.c_int, [_P, _P]),
     "gr_autotune": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gr_proposal_stats": (ctypes.c_int, [_P, _P, _P]),
+    "gr_evaluate": (ctypes.c_int, [_P, _P, _P]),
+    "gr_eval_counts": (ctypes.c_int, [_P, _P, _P]),
     "gr_design_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32, _I32, ctypes.POINTER(_P)]),
     "gr_design_create": (ctypes.c_int, [_I32, _I32, _I32, _P, _I32, _P, _P, ctypes.POINTER(_P)]),
     "gr_design_free": (None, [_P]),
@@ -198,6 +200,21 @@ def route_cost(cap, demand):
     wl = int(d.sum())
     of = int(np.maximum(d - cap, 0).sum())
     return {"wirelength": wl, "overflow": of, "total": wl + of}
+
+
+class _EvalLayer(ctypes.Structure):
+    _fields_ = [("wire", ctypes.c_int64), ("via", ctypes.c_int64), ("max_wire", ctypes.c_int32),
+                ("max_via", ctypes.c_int32), ("max_cap", ctypes.c_float), ("max_overflow", ctypes.c_double),
+                ("of_cost", ctypes.c_double), ("demand", ctypes.c_int64), ("overflow", ctypes.c_double)]
+
+
+class _EvalTotal(ctypes.Structure):
+    _fields_ = [("wire", ctypes.c_int64), ("via", ctypes.c_int64), ("of_cost", ctypes.c_double),
+                ("demand", ctypes.c_int64), ("overflow", ctypes.c_double)]
+
+
+def _as_dict(st):
+    return {k: getattr(st, k) for k, _ in st._fields_}
 
 
 FMT_FIRST_ACCESS_POINT, FMT_NEW = 1, 2
@@ -392,6 +409,21 @@ class Router:
         out = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
         self._c(self.lib.gr_get_demand(self.h, _ptr(out)))
         return out
+
+    def evaluate(self):
+        """Route report of the resident schedule's routes (gr_evaluate):
+        (per-layer dicts, totals dict)."""
+        layers = (_EvalLayer * self.Z)()
+        tot = _EvalTotal()
+        self._c(self.lib.gr_evaluate(self.h, layers, ctypes.byref(tot)))
+        return [_as_dict(l) for l in layers], _as_dict(tot)
+
+    def eval_counts(self):
+        """Per-cell (wire, via) counts of the last evaluate()."""
+        w = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
+        v = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
+        self._c(self.lib.gr_eval_counts(self.h, _ptr(w), _ptr(v)))
+        return w, v
 
     def remaining_f32(self):
         """Remaining fp32 capacity per cell (fp32 grids only)."""

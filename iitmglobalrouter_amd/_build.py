@@ -5,6 +5,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libiitmgr.so")
 SOURCES = [os.path.join(HERE, "csrc", "gr_route.hip"), os.path.join(HERE, "csrc", "gr_mlp.hip"),
+           os.path.join(HERE, "csrc", "gr_eval.hip"),
            os.path.join(HERE, "csrc", "gr_host.cpp"), os.path.join(HERE, "csrc", "gr_io.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "gr_kernels.h"), os.path.join(HERE, "csrc", "gr_internal.h"), os.path.join(HERE, "csrc", "gr_device.h"),
            os.path.join(HERE, "..", "include", "iitmgr.h")]

@@ -32,6 +32,7 @@ struct BatchMeta {
   int64_t P;              // pin pairs in the batch
   int32_t chunk_ofs;      // first entry of the batch in the pair-chunk table
   int32_t cls_ofs[kNumCls], cls_n[kNumCls];
+  int64_t slot0, nslots;  // output slots of the batch
   int32_t max_large;      // most pins of a large net in the batch
   int64_t max_large_P;    // most pin pairs of a large net in the batch
 };
@@ -72,6 +73,10 @@ struct gr_ctx {
   float *d_pfx = nullptr, *d_lut = nullptr;
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
   float *d_remf = nullptr, *d_capf0 = nullptr;  // fp32 grids: remaining and initial capacity
+  // route evaluator: per-cell wire and via counts, per-batch wire stamps, partial sums
+  int32_t *d_wcnt = nullptr, *d_vcnt = nullptr, *d_estamp = nullptr;
+  void* d_eval_part = nullptr;
+  void* d_eval_out = nullptr;
   size_t lds_max = 65536;
   gr_tuning tune = {1024, 64, 1, 2048, 0};
   int32_t n_nets = 0;

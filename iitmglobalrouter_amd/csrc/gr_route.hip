@@ -109,6 +109,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     int64_t P = 0;
     m.max_large = 0;
     m.max_large_P = 0;
+    m.slot0 = slots;
     for (int i = 0; i < m.n; ++i) {
       const int32_t id = net_ids[m.begin + i];
       if (id < 0 || id >= c->n_nets) return fail(c, GR_EINVAL, "net id out of range");
@@ -121,6 +122,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       slots += d;
       P += d * (d - 1) / 2;
     }
+    m.nslots = slots - m.slot0;
     for (int cls = 0; cls < kNumCls; ++cls) {
       m.cls_ofs[cls] = lo;
       for (int i = 0; i < m.n; ++i) {
@@ -414,6 +416,8 @@ void gr_destroy(gr_ctx* c) {
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_stamp); (void)hipFree(c->d_run_stamp);
   (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
+  (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
+  (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
   (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat);
   free_sched(c->sched);

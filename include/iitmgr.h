@@ -248,6 +248,43 @@ int gr_set_proposal(gr_ctx* ctx, gr_mlp* m, int32_t k, float threshold);
 /* Pairs costed and pairs that fell back since the last call (counters reset). */
 int gr_proposal_stats(gr_ctx* ctx, int64_t* pairs, int64_t* fallbacks);
 
+/* ---- route evaluation ---- */
+
+typedef struct {
+  int64_t wire;         /* sum over the layer's cells of the wire count */
+  int64_t via;          /* sum of the via count */
+  int32_t max_wire;     /* largest wire count of a cell (report starts at 0) */
+  int32_t max_via;      /* largest via count (the reference log prints it halved) */
+  float max_cap;        /* largest capacity (starts at 0) */
+  double max_overflow;  /* largest (2 wire + via - 2 cap) / 2, starting at 0 */
+  double of_cost;       /* sum of e^((2 wire + via - 2 cap) / 4), exponent rounded to fp32 */
+  int64_t demand;       /* sum of the committed demand (cells the router used) */
+  double overflow;      /* sum of max(0, demand - cap) */
+} gr_eval_layer;
+
+typedef struct {
+  int64_t wire, via;    /* layers 1..Z-1, as the report sums them */
+  double of_cost;       /* layers 1..Z-1 */
+  int64_t demand;       /* all layers */
+  double overflow;      /* all layers */
+} gr_eval_total;
+
+/*
+ * Evaluate the routes held in the resident schedule's outputs (after
+ * gr_route_schedule over the schedule) on the device.  Wire and via counts
+ * per cell follow the reference's non-BIT commit counters
+ * (mst_files/mst_commit.hpp:17-67 via charges, :68-113 and :329-341 wire
+ * cells [min, max) of the run, once per batch) along each edge's written
+ * route; the per-layer figures are those of the end-of-run report
+ * (main.cpp:103-135).  layers gets Z entries (may be NULL).
+ * Replaces the EVAL_OFCOST counters and report (main.cpp:103-135,
+ * read_data.hpp:162-168).
+ */
+int gr_evaluate(gr_ctx* ctx, gr_eval_layer* layers, gr_eval_total* total);
+
+/* Per-cell wire and via counts (CSIZE int32 each, either may be NULL) of the last gr_evaluate. */
+int gr_eval_counts(gr_ctx* ctx, int32_t* wire, int32_t* via);
+
 /* ---- host side of the path (no device work) ---- */
 
 /*

@@ -27,6 +27,8 @@ _SIGS = {
     "orc_set_remf": (ctypes.c_int, [_P, _P, _P]),
     "orc_cw_exp": (ctypes.c_double, [ctypes.c_double]),
     "orc_cost_f32": (ctypes.c_float, [ctypes.c_int, ctypes.c_float]),
+    "orc_evaluate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int, _P, _P, _P, _P,
+                                    _P, _P, _P, _P, _P]),
     "orc_destroy": (None, [_P]),
     "orc_set_nets": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "orc_get_demand": (ctypes.c_int, [_P, _P]),
@@ -93,6 +95,26 @@ def cw_exp(x):
 def cost_f32(model, r):
     """Cell cost of fp32 remaining capacity r under a cost model."""
     return np.float32(lib().orc_cost_f32(int(model), float(np.float32(r))))
+
+
+EVAL_FIELDS = ("wire", "via", "max_wire", "max_via", "max_cap", "max_overflow", "of_cost", "demand", "overflow")
+
+
+def evaluate(X, Y, Z, cap, demand, batch_ofst, net_ids, pin_pfx, pins, edges, lpat):
+    """Route evaluation: (per-cell wire counts, via counts, per-layer dict list)."""
+    cs = X * Y * Z
+    w = np.empty(cs, np.int32)
+    v = np.empty(cs, np.int32)
+    out = np.empty(9 * Z, np.float64)
+    args = [np.ascontiguousarray(cap, np.float32), np.ascontiguousarray(demand, np.int32)]
+    ofst = np.ascontiguousarray(batch_ofst, np.int32)
+    rest = [ofst, np.ascontiguousarray(net_ids, np.int32), np.ascontiguousarray(pin_pfx, np.int64),
+            np.ascontiguousarray(pins, np.int32), np.ascontiguousarray(edges, np.int64),
+            np.ascontiguousarray(lpat, np.int32)]
+    lib().orc_evaluate(X, Y, Z, _ptr(args[0]), _ptr(args[1]), len(ofst) - 1, *[_ptr(a) for a in rest],
+                       _ptr(w), _ptr(v), _ptr(out))
+    layers = [dict(zip(EVAL_FIELDS, out[9 * z:9 * z + 9])) for z in range(Z)]
+    return w, v, layers
 
 
 class Oracle:

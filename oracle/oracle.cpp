@@ -391,6 +391,111 @@ int orc_set_remf(void* h, const float* remf, const int32_t* demand) {
   return 0;
 }
 
+// Route evaluation (main.cpp:103-135 over the non-BIT commit's counters).
+// For every batch, the wires of its routes go into a difference array
+// (+1 at the run's lower end cell, -1 at the upper, gpu_commit_wire
+// mst_commit.hpp:68-113), every run is scanned inclusively
+// (get_gpu_pat_ipfx :252-328) and each cell with a positive sum counts one
+// wire (update_cap_cube :329-341); vias charge their layer's run neighbours
+// (gpu_commit_via :17-61).  Routes are walked as the writer prints them
+// (writer_util.hpp:115-165).  Then the report loop of main.cpp:109-131 runs
+// per layer in its x-then-y order, summing in double.  layer_out gets 9
+// doubles per layer: wire, via, max_wire, max_via, max_cap, max_overflow,
+// of_cost, demand, overflow.
+int orc_evaluate(int X, int Y, int Z, const float* cap, const int32_t* demand, int n_batches,
+                 const int32_t* batch_ofst, const int32_t* net_ids, const int64_t* pin_pfx, const int32_t* pins,
+                 const int64_t* edges, const int32_t* lpat, int32_t* wcnt, int32_t* vcnt, double* layer_out) {
+  const int64_t cs = (int64_t)X * Y * Z;
+  const int Z2 = (Z + 1) / 2;
+  auto enc = [&](int z, int x, int y) -> int64_t {
+    return (z & 1) ? (int64_t)z * X * Y + (int64_t)x * Y + y : (int64_t)z * X * Y + (int64_t)y * X + x;
+  };
+  std::fill(wcnt, wcnt + cs, 0);
+  std::fill(vcnt, vcnt + cs, 0);
+  std::vector<int32_t> diff(cs);
+  int64_t slot = 0;
+  for (int b = 0; b < n_batches; ++b) {
+    std::fill(diff.begin(), diff.end(), 0);
+    for (int q = batch_ofst[b]; q < batch_ofst[b + 1]; ++q) {
+      const int net = net_ids[q];
+      const int64_t deg = pin_pfx[net + 1] - pin_pfx[net];
+      const int32_t* np = pins + pin_pfx[net];
+      for (int64_t j = 0; j < deg; ++j) {
+        const int64_t e = edges[slot + j];
+        if (e == -1) continue;
+        int hi = (int)(e & 0x7fff), lo = (int)((e >> 15) & 0x7fff);
+        if (hi < lo) std::swap(hi, lo);
+        int z1, x1, y1, z2, x2, y2;
+        auto dec = [&](int64_t v, int& z, int& x, int& y) {
+          z = (int)(v / ((int64_t)X * Y));
+          const int64_t r = v - (int64_t)z * X * Y;
+          if (z & 1) { x = (int)(r / Y); y = (int)(r % Y); } else { y = (int)(r / X); x = (int)(r % X); }
+        };
+        dec(np[hi], z1, x1, y1);
+        dec(np[lo], z2, x2, y2);
+        const int p = lpat[slot + j];
+        const int via1 = p / Z2;
+        int via2 = p % Z2;
+        const bool both = x1 != x2 && y1 != y2;
+        auto vias = [&](int za, int zb) {
+          for (int i = std::min(za, zb); i < std::max(za, zb); ++i) {
+            const int64_t c = enc(i, x1, y1);
+            const int pos = (i & 1) ? y1 : x1, L = (i & 1) ? Y : X;
+            if (pos == 0) vcnt[c] += 2;
+            else if (pos == L - 1) vcnt[c - 1] += 2;
+            else { vcnt[c - 1] += 1; vcnt[c] += 1; }
+          }
+        };
+        auto wire = [&](int z) {
+          int64_t c1, c2;
+          if (z & 1) { c1 = enc(z, x1, std::min(y1, y2)); c2 = enc(z, x1, std::max(y1, y2)); y1 = y2; }
+          else       { c1 = enc(z, std::min(x1, x2), y1); c2 = enc(z, std::max(x1, x2), y1); x1 = x2; }
+          diff[c1] += 1;
+          diff[c2] -= 1;
+        };
+        if (z1 != via1) { vias(z1, via1); z1 = via1; }
+        if (z1 > 0) wire(z1);
+        via2 = (z1 & 1) ? 2 * via2 : 2 * via2 + 1;
+        if (both && via2 != 0 && z1 != via2) { vias(z1, via2); z1 = via2; wire(z1); }
+        if (z1 != z2) vias(z1, z2);
+      }
+      slot += deg;
+    }
+    for (int z = 0; z < Z; ++z) {
+      const int L = (z & 1) ? Y : X, lines = (z & 1) ? X : Y;
+      for (int l = 0; l < lines; ++l) {
+        int32_t run = 0;
+        const int64_t base = (int64_t)z * X * Y + (int64_t)l * L;
+        for (int t = 0; t < L; ++t) {
+          run += diff[base + t];
+          if (run > 0) wcnt[base + t] += 1;
+        }
+      }
+    }
+  }
+  for (int z = 0; z < Z; ++z) {
+    double w = 0, v = 0, mw = 0, mv = 0, mc = 0, mo = 0, oc = 0, dm = 0, ov = 0;
+    for (int x = 0; x < X; ++x)
+      for (int y = 0; y < Y; ++y) {
+        const int64_t c = enc(z, x, y);
+        const double cw = wcnt[c], cv = vcnt[c], cc = cap[c];
+        mw = std::max(mw, cw);
+        mv = std::max(mv, cv);
+        mc = std::max(mc, cc);
+        const double of = 2 * cw + cv - 2 * cc;
+        mo = std::max(mo, of / 2);
+        oc += (double)(float)cw_exp((double)(float)(of * 0.25));
+        w += cw;
+        v += cv;
+        dm += demand[c];
+        ov += std::max(0.0, (double)demand[c] - cc);
+      }
+    double* o = layer_out + 9 * z;
+    o[0] = w; o[1] = v; o[2] = mw; o[3] = mv; o[4] = mc; o[5] = mo; o[6] = oc; o[7] = dm; o[8] = ov;
+  }
+  return 0;
+}
+
 double orc_cw_exp(double x) { return cw_exp(x); }
 float orc_cost_f32(int model, float r) { return cost_f32(model, r); }
 
