@@ -72,6 +72,63 @@ def cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, budget_s):
                       f"C++ oracle with OpenMP over nets, {dt:.1f} s"}
 
 
+def io_eval(gr, r, X, Y, Z, cap, pfx, pins, ofst, ids, cpu=True):
+    """The path's callers either side, on the routes of the last timed pass:
+    route evaluator (device) with the oracle's CPU evaluation beside it,
+    route writer and design reader/saver (host, all threads and one thread).
+    The reader and writer are host code in the reference too (one reader
+    thread, read_data.hpp; four writer threads, writer.hpp)."""
+    import tempfile
+    out = {}
+    # evaluator: per-layer report of main.cpp:103-135 on the device
+    r.sync()
+    r.evaluate()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        layers, tot = r.evaluate()
+    ev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    cells = X * Y * Z
+    E, L, _ = r.schedule_outputs()
+    ev = {"ms": round(ev_ms, 3), "cells": cells, "slots": int(len(E)), "total": tot,
+          "note": "gr_evaluate end to end (clears, one path launch per batch, layer reduction, D2H)"}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.evaluate(X, Y, Z, np.asarray(cap, np.float32), r.demand(), ofst, ids, pfx, pins, E, L)
+        ev["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        ev["cpu_kind"] = "port: oracle difference-array evaluator, 1 thread"
+    out["evaluator"] = ev
+    d = gr.Design(X, Y, Z, np.asarray(cap, np.float32), pfx, pins)
+    with tempfile.TemporaryDirectory() as tmp:
+        wr = {}
+        for flags, name in ((0, "grid"), (gr.WRITE_MERGE | gr.WRITE_DBU, "merged_dbu")):
+            path = os.path.join(tmp, f"routes_{name}.txt")
+            for threads in (0, 1):
+                t0 = time.perf_counter()
+                nbytes = d.write_routes(path, ofst, ids, E, L, flags=flags, threads=threads)
+                dt = time.perf_counter() - t0
+                key = f"{name}_{'all' if threads == 0 else '1'}_threads"
+                wr[key] = {"s": round(dt, 3), "MB": round(nbytes / 1e6, 1), "MBps": round(nbytes / 1e6 / dt, 1),
+                           "nets_per_s": round(len(ids) / dt, 1)}
+        out["writer"] = wr
+        capf, netf = os.path.join(tmp, "d.cap"), os.path.join(tmp, "d.net")
+        t0 = time.perf_counter()
+        d.save(capf, netf)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(capf) + os.path.getsize(netf)
+        rd = {"MB": round(size / 1e6, 1), "save_s": round(save_s, 3)}
+        for threads in (0, 1):
+            t0 = time.perf_counter()
+            e = gr.Design.read(capf, netf, 0, threads)
+            dt = time.perf_counter() - t0
+            rd[f"{'all' if threads == 0 else '1'}_threads"] = {"s": round(dt, 3), "MBps": round(size / 1e6 / dt, 1)}
+            e.close()
+        out["reader"] = rd
+    out["threads_all"] = min(os.cpu_count() or 1, 16)
+    return out
+
+
 def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
     """Learned proposal on the same generator: samples from exhaustive routing of
     seeds 1..train_seeds train the MLP on the GPU; seed 0 is then routed with the
@@ -149,6 +206,7 @@ def main():
     ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
     ap.add_argument("--autotune", type=int, default=1, help="pick schedule parameters by timing (0: defaults)")
     ap.add_argument("--tuning", default="", help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave")
+    ap.add_argument("--io", type=int, default=1, help="also measure evaluator, writer and reader (0: skip)")
     ap.add_argument("--train-seeds", type=int, default=20)
     ap.add_argument("--train-per-batch", type=int, default=1500)
     ap.add_argument("--train-epochs", type=int, default=4)
@@ -241,6 +299,10 @@ def main():
         if os.path.exists(prof):
             with open(prof) as f:
                 roof["traffic"] = json.load(f)["per_launch"][KNAMES[dom]]["traffic_bytes"]
+    io = None
+    if args.io:
+        io = io_eval(gr, r, X, Y, Z, cap, pfx, pins, ofst, ids, cpu=not args.no_cpu)
+        print(f"# io {io}", file=sys.stderr, flush=True)
     learned = None
     if args.learned:
         learned = learned_eval(gr, r, X, Y, Z, cap, W["nets"], args, n_nets, ms_step, (pfx, pins, ofst, ids))
@@ -260,6 +322,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kstats,
         "learned": learned,
+        "callers": io,
         "path_bytes_per_step": bytes_path,
         "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
     }

@@ -45,10 +45,14 @@ __device__ __forceinline__ void via_charge(const Geo& g, int32_t* __restrict__ v
   else { atomicAdd(&vcnt[cell - 1], 1); atomicAdd(&vcnt[cell], 1); }
 }
 
-// One launch per batch; one lane per output slot of the batch.  A wire cell
-// is counted by the lane whose exchange of the batch stamp finds another
-// value; launches of one stream run in order, so each (cell, batch) counts
-// once and no two lanes of a launch update the same wire count.
+// One launch per batch; one lane per output slot of the batch.  Each lane
+// walks its edge's route: via charges go out at once (no-return atomics), the
+// up to two wires are kept as cell ranges.  The wavefront then spreads all
+// its wire cells over its 64 lanes (prefix of lengths, owner found by a
+// shuffle binary search), so every step issues 64 independent exchanges.  A
+// wire cell is counted by the lane whose exchange of the batch stamp finds
+// another value; launches of one stream run in order, so each (cell, batch)
+// counts once and no two lanes of a launch update the same wire count.
 __global__ __launch_bounds__(256) void k_eval_paths(Geo g, const int64_t* __restrict__ edges,
                                                     const int32_t* __restrict__ lpat, int64_t s0, int64_t ns,
                                                     const int64_t* __restrict__ pinofs,
@@ -57,45 +61,76 @@ __global__ __launch_bounds__(256) void k_eval_paths(Geo g, const int64_t* __rest
                                                     int32_t* __restrict__ vcnt, int32_t* __restrict__ estamp,
                                                     int32_t stamp) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (j >= ns) return;
-  const int64_t e = edges[s0 + j];
-  if (e < 0) return;
-  const int local = (int)(e >> 30);
-  if (local >= npos) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t st0 = 0, st1 = 0;  // first cell of each wire
+  int len0 = 0, len1 = 0;     // cells of each wire, [min, max) of its run
+  bool ok = j < ns;
+  int64_t e = ok ? edges[s0 + j] : -1;
+  ok = ok && e >= 0;
+  const int local = ok ? (int)(e >> 30) : 0;
+  ok = ok && local < npos;
   int n1 = (int)(e & 0x7fff), n2 = (int)((e >> 15) & 0x7fff);
   if (n1 < n2) { const int t = n1; n1 = n2; n2 = t; }
-  if (n1 >= deg[local]) return;
-  const int64_t po = pinofs[local];
-  int z1, x1, y1, z2, x2, y2;
-  dec(g, (uint32_t)pins[po + n1], z1, x1, y1);
-  dec(g, (uint32_t)pins[po + n2], z2, x2, y2);
-  const int p = lpat[s0 + j];
-  if (p < 0 || p >= g.Z * g.Z2) return;
-  const int a = p / g.Z2;
-  int b = p - a * g.Z2;
-  const bool both = x1 != x2 && y1 != y2;
-  auto vias = [&](int za, int zb) {
-    const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
-    for (int l = lo; l < hi; ++l) via_charge(g, vcnt, l, x1, y1);
-  };
-  auto wire = [&](int z) {
-    const bool odd = z & 1;
-    const int u = odd ? y1 : x1, v = odd ? y2 : x2;
-    const int line = odd ? x1 : y1, L = odd ? g.Y : g.X;
-    const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
-    for (int t = min(u, v); t < max(u, v); ++t)
-      if (atomicExch(&estamp[base + t], stamp) != stamp) wcnt[base + t] += 1;
-    if (odd) y1 = y2; else x1 = x2;
-  };
-  if (z1 != a) { vias(z1, a); z1 = a; }
-  if (z1 > 0) wire(z1);
-  b = (z1 & 1) ? 2 * b : 2 * b + 1;
-  if (both && b != 0 && z1 != b && b < g.Z) {
-    vias(z1, b);
-    z1 = b;
-    wire(z1);
+  ok = ok && n1 < deg[local];
+  const int p = ok ? lpat[s0 + j] : -1;
+  ok = ok && p >= 0 && p < g.Z * g.Z2;
+  if (ok) {
+    const int64_t po = pinofs[local];
+    int z1, x1, y1, z2, x2, y2;
+    dec(g, (uint32_t)pins[po + n1], z1, x1, y1);
+    dec(g, (uint32_t)pins[po + n2], z2, x2, y2);
+    const int a = p / g.Z2;
+    int b = p - a * g.Z2;
+    const bool both = x1 != x2 && y1 != y2;
+    auto vias = [&](int za, int zb) {
+      const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
+      for (int l = lo; l < hi; ++l) via_charge(g, vcnt, l, x1, y1);
+    };
+    auto wire = [&](int z, uint32_t& st, int& len) {
+      const bool odd = z & 1;
+      const int u = odd ? y1 : x1, v = odd ? y2 : x2;
+      const int line = odd ? x1 : y1, L = odd ? g.Y : g.X;
+      st = (uint32_t)z * g.XY + (uint32_t)line * L + (uint32_t)min(u, v);
+      len = max(u, v) - min(u, v);
+      if (odd) y1 = y2; else x1 = x2;
+    };
+    if (z1 != a) { vias(z1, a); z1 = a; }
+    if (z1 > 0) wire(z1, st0, len0);
+    b = (z1 & 1) ? 2 * b : 2 * b + 1;
+    if (both && b != 0 && z1 != b && b < g.Z) {
+      vias(z1, b);
+      z1 = b;
+      wire(z1, st1, len1);
+    }
+    if (z1 != z2) vias(z1, z2);
   }
-  if (z1 != z2) vias(z1, z2);
+  // inclusive prefix of wire cells over the wavefront
+  int incl = len0 + len1;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  for (int k0 = 0; k0 < total; k0 += 64) {
+    const int k = k0 + lane;
+    // owner: the first lane whose inclusive prefix exceeds k
+    int lo = 0, hi = 63;
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+      const int mid = (lo + hi) >> 1;
+      if (__shfl(incl, mid, 64) > k) hi = mid; else lo = mid + 1;
+    }
+    const int o = lo;
+    const int before = __shfl(incl, o, 64) - __shfl(len0 + len1, o, 64);
+    const int l0 = __shfl(len0, o, 64);
+    const uint32_t a0 = (uint32_t)__shfl((int)st0, o, 64), a1 = (uint32_t)__shfl((int)st1, o, 64);
+    if (k < total) {
+      const int off = k - before;
+      const uint32_t cell = off < l0 ? a0 + off : a1 + (off - l0);
+      if (atomicExch(&estamp[cell], stamp) != stamp) wcnt[cell] += 1;
+    }
+  }
 }
 
 __device__ __forceinline__ void merge(EvalPart& a, const EvalPart& b) {
