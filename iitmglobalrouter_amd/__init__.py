@@ -16,7 +16,7 @@ __all__ = [
     "load_library", "RouterError", "Router", "Model", "Tuning", "Design", "route_cost", "gen_grid", "gen_nets",
     "partition_nets", "FMT_FIRST_ACCESS_POINT", "FMT_NEW", "WRITE_DBU", "WRITE_MERGE", "WRITE_PATCH",
     "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
-    "WORKLOADS", "device_count",
+    "WORKLOADS", "device_count", "TREE_ROOT_PAIR", "TREE_TRUE_MST",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "libiitmgr.so")
 
 COST_REF_BIT, COST_REF_INC, COST_LIT_EXP = 0, 1, 2
 COMMIT_REF_QUIRK, COMMIT_FIXED = 0, 1
+TREE_ROOT_PAIR, TREE_TRUE_MST = 0, 1
 K_LPAT, K_TREE, K_REFRESH = 0, 1, 2
 
 # Workload families of the synthetic generator: grid shape and net count.
@@ -108,6 +109,7 @@ _SIGS = {
     "gr_autotune": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gr_proposal_stats": (ctypes.c_int, [_P, _P, _P]),
     "gr_evaluate": (ctypes.c_int, [_P, _P, _P]),
+    "gr_set_tree_mode": (ctypes.c_int, [_P, _I32]),
     "gr_eval_counts": (ctypes.c_int, [_P, _P, _P]),
     "gr_design_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I32, _I32, ctypes.POINTER(_P)]),
     "gr_design_create": (ctypes.c_int, [_I32, _I32, _I32, _P, _I32, _P, _P, ctypes.POINTER(_P)]),
@@ -409,6 +411,10 @@ class Router:
         out = np.empty(self.X * self.Y * self.Z, dtype=np.int32)
         self._c(self.lib.gr_get_demand(self.h, _ptr(out)))
         return out
+
+    def set_tree_mode(self, true_mst):
+        """False: root-pair trees (the reference); True: minimum spanning trees."""
+        self._c(self.lib.gr_set_tree_mode(self.h, TREE_TRUE_MST if true_mst else TREE_ROOT_PAIR))
 
     def evaluate(self):
         """Route report of the resident schedule's routes (gr_evaluate):

@@ -67,6 +67,7 @@ struct gr_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   grd::Geo g{};
   int model = 0, quirk = 1, M = 0;
+  int true_mst = 0;  // GR_TREE_TRUE_MST
   int64_t cs = 0;
   float lut[grd::kLut];
   int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;

@@ -268,7 +268,19 @@ struct TreeArgs {
   int32_t bstamp;      // filled from *ctr at kernel entry
   int32_t quirk;
   int32_t M;  // run_stamp row stride: max(X, Y)
+  int32_t true_mst;  // record each accepted component's minimum pair instead of the root pair
 };
+
+// The edge an accepted candidate of component root r records: the root pair
+// (r, partner root) as the reference does (gmst_util.hpp:245, 312, 322), or
+// in true-MST mode the minimum pair itself, its endpoint in r's component
+// first.  root[] must still hold the round's starting roots.
+__device__ __forceinline__ int2 recorded_edge(const TreeArgs& A, const int* root, int r, int pr, uint32_t o) {
+  if (!A.true_mst) return make_int2(r, pr);
+  int j, k;
+  pair_decode(o, j, k);
+  return root[j] == r ? make_int2(j, k) : make_int2(k, j);
+}
 
 // Commit marks are idempotent plain stores: the cell's stamp and its run's
 // stamp.  The demand update itself happens once per stamped cell in k_refresh.
@@ -345,7 +357,8 @@ __device__ __forceinline__ void emit_edge(const TreeArgs& A, int pos, int64_t sb
 
 // 2-pin nets: the only pair is (1, 0); Boruvka's single round
 // has the mutual candidates 0 -> 1 and 1 -> 0, and root 0 is processed first,
-// so the edge is (0, 1) and the tree cost is 0 + cost.
+// so the edge is (0, 1) and the tree cost is 0 + cost (in true-MST mode the
+// minimum pair's endpoints are the roots themselves: the same edge).
 __device__ void tree2_commit(const TreeArgs& A, int pos) {
   const int64_t pb = A.pinofs[pos];
   const int64_t sb = A.slot[pos];
@@ -506,7 +519,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
         int off = ne;
         for (int q = 0; q < it * nwv + wv; ++q) off += wcnt[q];
         off += __popcll(m & ((1ull << lane) - 1ull));
-        edges[off] = make_int2(r, pr);
+        edges[off] = recorded_edge(A, root, r, pr, (uint32_t)best[r]);
         ecost[off] = cost[(uint32_t)best[r]];
         dsu_union(par, r, pr, 4 * n);
       }
@@ -610,7 +623,7 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     const unsigned long long gm = __ballot(acc) & gbits;
     if (acc) {
       const int off = ne + __popcll(gm & ((1ull << lane) - 1ull));
-      edges[off] = make_int2(tid, pr);
+      edges[off] = recorded_edge(A, root, tid, pr, (uint32_t)best[tid]);
       ecost[off] = cost[(uint32_t)best[tid]];
       dsu_union(link, tid, pr, 4 * G);
     }

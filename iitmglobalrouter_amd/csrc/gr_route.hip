@@ -246,6 +246,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.bstamp = 0;
   A.quirk = c->quirk;
   A.M = c->M;
+  A.true_mst = c->true_mst;
   {
     KTimer t(c, GR_K_TREE);
     SmallClasses C;
@@ -621,6 +622,18 @@ int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
     const std::vector<int32_t> ofst = s.h_ofst, ids = s.h_ids;
     return build_sched(c, s, (int32_t)ofst.size() - 1, ofst.data(), ids.data());
   }
+  return 0;
+}
+
+int gr_set_tree_mode(gr_ctx* c, int32_t mode) {
+  if (!c) return fail(c, GR_EINVAL, "null context");
+  if (mode != GR_TREE_ROOT_PAIR && mode != GR_TREE_TRUE_MST) return fail(c, GR_EINVAL, "unknown tree mode");
+  (void)hipSetDevice(c->dev);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->true_mst = mode == GR_TREE_TRUE_MST;
+  Sched& s = c->sched;
+  if (s.exec) { (void)hipGraphExecDestroy(s.exec); s.exec = nullptr; }  // the graph holds the old mode
+  s.graph_failed = false;
   return 0;
 }
 

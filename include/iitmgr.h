@@ -55,6 +55,10 @@ extern "C" {
 #define GR_COMMIT_REF_QUIRK 0 /* reproduce the unguarded first wire, mst_bit_commit.hpp:216 */
 #define GR_COMMIT_FIXED 1     /* mark only the cells of the route */
 
+/* tree modes */
+#define GR_TREE_ROOT_PAIR 0  /* record the component-root pair (gmst_util.hpp:245, 312, 322), default */
+#define GR_TREE_TRUE_MST 1   /* record each component's minimum pair: a true MST */
+
 typedef struct gr_ctx gr_ctx;
 
 typedef struct {
@@ -137,6 +141,18 @@ int gr_set_schedule(gr_ctx* ctx, int32_t n_batches, const int32_t* batch_ofst,
  * gr_route_batch.  Replaces the router thread's while loop, test_lpat.hpp:280.
  */
 int gr_route_schedule(gr_ctx* ctx, int32_t first, int32_t count);
+
+/*
+ * Tree mode for the following batches.  GR_TREE_ROOT_PAIR reproduces the
+ * reference: a component that picks its minimum outgoing pair records the
+ * pair of component roots, and that pair's L-pattern is committed.
+ * GR_TREE_TRUE_MST records the minimum pair itself (same Boruvka rounds,
+ * same lowest-index ties), so each net's tree is its minimum spanning tree
+ * over the pair costs, the tree the reference's CPU Kruskal twin builds
+ * (MST_UTIL::t_get_mst, mst_util.hpp:34-102).  net_cost is the same sum in
+ * both modes; the committed routes differ.
+ */
+int gr_set_tree_mode(gr_ctx* ctx, int32_t mode);
 
 /* Wait for all enqueued work; returns a HIP error as GR_ENODEV. */
 int gr_sync(gr_ctx* ctx);

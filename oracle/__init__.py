@@ -38,7 +38,8 @@ _SIGS = {
     "orc_pattern_cost_walk": (ctypes.c_double, [_P, _I32, _I32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "orc_best_pattern": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
     "orc_edge_cells": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int, _P, ctypes.c_int]),
-    "orc_boruvka": (ctypes.c_int, [ctypes.c_int, _P, _P, _P]),
+    "orc_boruvka": (ctypes.c_int, [ctypes.c_int, _P, _P, _P, ctypes.c_int]),
+    "orc_set_tree_mode": (None, [_P, ctypes.c_int]),
     "orc_route_batch": (_I64, [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]),
     "orc_last_pairs": (_I64, [_P]),
     "orc_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
@@ -76,12 +77,12 @@ def lut(model):
     return out
 
 
-def boruvka(n, costs):
-    """Root-pair Boruvka on explicit pair costs: (edges[(on1, on2)], net_cost, rounds)."""
+def boruvka(n, costs, true_mst=False):
+    """Root-pair (or true-MST) Boruvka on explicit pair costs: (edges[(on1, on2)], net_cost, rounds)."""
     c = np.ascontiguousarray(costs, dtype=np.float32)
     out = np.empty(n, dtype=np.int64)
     nc = ctypes.c_float()
-    rounds = lib().orc_boruvka(n, _ptr(c), _ptr(out), ctypes.byref(nc))
+    rounds = lib().orc_boruvka(n, _ptr(c), _ptr(out), ctypes.byref(nc), int(true_mst))
     edges = [(int(v >> 15), int(v & 0x7FFF)) for v in out[: n - 1]]
     assert out[n - 1] == -1
     return edges, nc.value, rounds
@@ -147,6 +148,9 @@ class Oracle:
         self.pin_enc = np.ascontiguousarray(pin_enc, dtype=np.int32)
         self.deg = np.diff(self.pin_pfx)
         lib().orc_set_nets(self.h, len(self.pin_pfx) - 1, _ptr(self.pin_pfx), _ptr(self.pin_enc))
+
+    def set_tree_mode(self, true_mst):
+        lib().orc_set_tree_mode(self.h, int(bool(true_mst)))
 
     def set_rem(self, rem):
         r = np.ascontiguousarray(rem, dtype=np.int32)

@@ -15,6 +15,8 @@
 //   * pattern code p = a*Z2 + b          lpattern_util.hpp:43-53
 //   * pair order o = j(j-1)/2 + k        test_lpat.hpp:311-331, gmst_util.hpp:192-194
 //   * root-pair Boruvka                  gmst_util.hpp:164-341, mst_util.hpp:289-314
+//     (true-MST mode records the minimum pairs: the tree of the CPU twin
+//     MST_UTIL::t_get_mst, mst_util.hpp:34-102)
 //   * BIT commit cell marking            mst_bit_commit.hpp:17-67, 81-235
 //   * per-batch demand update            bit_util.hpp:75-90
 //
@@ -110,6 +112,7 @@ inline uint32_t ord_key(float f) {
 
 struct Ctx {
   int X, Y, Z, Z2, model, commit_mode;
+  bool true_mst = false;
   int64_t cs;
   std::vector<int32_t> cap0, rem, stamp;
   bool fmode = false;       // fp32 capacities: remf is the state, rem counts -demand
@@ -282,10 +285,12 @@ struct Ctx {
 
 // Root-pair Boruvka on one net, gmst_util.hpp:164-341 with the lowest-index
 // tie contract.  costs[o] for o = j(j-1)/2 + k.  Emits (on1, on2) pairs where
-// on1 is the selecting component's root at round start and on2 the partner's.
+// on1 is the selecting component's root at round start and on2 the partner's;
+// with true_mst the selecting component's minimum pair itself, its endpoint
+// in that component first (the rounds and unions are the same).
 int boruvka(int n, const float* costs, std::vector<std::pair<int, int>>& edges,
-            std::vector<int>& eo, float& net_cost) {
-  std::vector<int> dsu(n), partner(n);
+            std::vector<int>& eo, float& net_cost, bool true_mst = false) {
+  std::vector<int> dsu(n), partner(n), end_in(n), end_out(n);
   std::vector<uint64_t> best(n);
   for (int i = 0; i < n; ++i) dsu[i] = i;
   auto find = [&](int v) {
@@ -323,6 +328,8 @@ int boruvka(int n, const float* costs, std::vector<std::pair<int, int>>& edges,
       int k = (int)(bo - (int64_t)j * (j - 1) / 2);
       int rj = find(j), rk = find(k);
       partner[r] = (rj == r) ? rk : rj;
+      end_in[r] = (rj == r) ? j : k;
+      end_out[r] = (rj == r) ? k : j;
     }
     // serial union in ascending root order (gmst_util.hpp:296-341)
     for (int r = 0; r < n; ++r) {
@@ -331,7 +338,8 @@ int boruvka(int n, const float* costs, std::vector<std::pair<int, int>>& edges,
       if (n1 == n2) continue;
       if (n1 < n2) dsu[n1] = n2; else dsu[n2] = n1;
       net_cost += costs[best[r] & 0xffffffffu];
-      edges.emplace_back(r, partner[r]);
+      if (true_mst) edges.emplace_back(end_in[r], end_out[r]);
+      else edges.emplace_back(r, partner[r]);
       eo.push_back((int)(best[r] & 0xffffffffu));
     }
   }
@@ -496,6 +504,8 @@ int orc_evaluate(int X, int Y, int Z, const float* cap, const int32_t* demand, i
   return 0;
 }
 
+void orc_set_tree_mode(void* h, int true_mst) { ((Ctx*)h)->true_mst = true_mst != 0; }
+
 double orc_cw_exp(double x) { return cw_exp(x); }
 float orc_cost_f32(int model, float r) { return cost_f32(model, r); }
 
@@ -586,11 +596,11 @@ int orc_edge_cells(void* h, int32_t src, int32_t dst, int p, int64_t* out, int c
 
 // Boruvka on an explicit cost array (local net 0).  out_edges gets n slots:
 // n-1 encoded edges then -1; returns the number of rounds.
-int orc_boruvka(int n, const float* costs, int64_t* out_edges, float* out_cost) {
+int orc_boruvka(int n, const float* costs, int64_t* out_edges, float* out_cost, int true_mst) {
   std::vector<std::pair<int, int>> e;
   std::vector<int> eo;
   float nc;
-  int rounds = boruvka(n, costs, e, eo, nc);
+  int rounds = boruvka(n, costs, e, eo, nc, true_mst != 0);
   for (size_t i = 0; i < e.size(); ++i)
     out_edges[i] = ((int64_t)e[i].first << 15) | (int64_t)e[i].second;
   out_edges[n - 1] = -1;
@@ -631,7 +641,7 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
     std::vector<std::pair<int, int>> e;
     std::vector<int> eo;
     float nc;
-    boruvka(d, pc, e, eo, nc);
+    boruvka(d, pc, e, eo, nc, c->true_mst);
     out_net_cost[i] = nc;
     int64_t base = slot_pfx[i];
     for (size_t t = 0; t < e.size(); ++t) {
