@@ -299,6 +299,28 @@ def main():
         if os.path.exists(prof):
             with open(prof) as f:
                 roof["traffic"] = json.load(f)["per_launch"][KNAMES[dom]]["traffic_bytes"]
+    # side figure: the same pass with minimum spanning trees (GR_TREE_TRUE_MST)
+    # instead of the reference's root-pair trees, for route quality
+    rp_cost = gr.route_cost(cap, r.demand())
+    r.set_tree_mode(True)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset_demand()
+        r.route_schedule()
+    r.sync()
+    mst_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    mst_cost = gr.route_cost(cap, r.demand())
+    true_mst = {"ms_per_step": round(mst_ms, 4), "nets_per_s": round(n_nets / (mst_ms / 1e3), 1),
+                "route_cost": mst_cost, "root_pair_route_cost": rp_cost,
+                "route_cost_change": round((mst_cost["total"] - rp_cost["total"]) / rp_cost["total"], 5)}
+    r.set_tree_mode(False)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    print(f"# true-MST trees {true_mst}", file=sys.stderr, flush=True)
     io = None
     if args.io:
         io = io_eval(gr, r, X, Y, Z, cap, pfx, pins, ofst, ids, cpu=not args.no_cpu)
@@ -322,6 +344,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kstats,
         "learned": learned,
+        "true_mst": true_mst,
         "callers": io,
         "path_bytes_per_step": bytes_path,
         "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
