@@ -70,7 +70,8 @@ struct gr_ctx {
   int true_mst = 0;  // GR_TREE_TRUE_MST
   int64_t cs = 0;
   float lut[grd::kLut];
-  int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_stamp = nullptr, *d_run_stamp = nullptr;
+  int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_run_stamp = nullptr;
+  uint8_t* d_mark = nullptr;  // per cell: marked by this batch's commit
   float *d_pfx = nullptr, *d_lut = nullptr;
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
   float *d_remf = nullptr, *d_capf0 = nullptr;  // fp32 grids: remaining and initial capacity

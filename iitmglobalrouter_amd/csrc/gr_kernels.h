@@ -8,13 +8,14 @@
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
-//   stamp[cell]      int32  batch stamp of the last commit that used the cell
+//   mark[cell]       uint8  1 from the commit that used the cell until the
+//                           same batch's refresh consumes it
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
 // Per batch:
 //   k_lpat_lane  one lane per pin pair evaluates every L-pattern serially
 //   k_tree_small/_large  root-pair Boruvka + commit: one lane per 2-pin net,
 //                one wavefront per net up to 64 pins, one workgroup above
-//   k_refresh    per dirty run: demand -1 on stamped cells, canonical Fenwick
+//   k_refresh    per dirty run: demand -1 on marked cells, canonical Fenwick
 //                nodes rebuilt in LDS, run's pfx rewritten
 #pragma once
 #include "gr_device.h"
@@ -27,19 +28,31 @@ namespace grd {
 
 // Runs are scanned by a persistent grid; run id r = z*M + line.  all != 0
 // rebuilds every run without touching demand (set-up and reset).  Otherwise a
-// run is processed only when the batch's commit stamped it: its stamped cells
+// run is processed only when the batch's commit stamped it: its marked cells
 // lose one unit of capacity (bit_util.hpp:75-90 applies cap -= 1 once per
-// flagged cell), then Fenwick node i (1-based, lowbit 2^k, covering cells
-// [i - 2^k, i)) is the pairwise fp32 sum of its aligned block, built level by
-// level in LDS, and Q(x) accumulates nodes in the reference's descent order
+// flagged cell) and their marks are cleared, then Fenwick node i (1-based,
+// lowbit 2^k, covering cells [i - 2^k, i)) is the pairwise fp32 sum of its
+// aligned block, and Q(x) accumulates nodes in the reference's descent order
 // (bit_util.hpp:19-35).
 //
-// FLT: grids with fp32 capacities.  remf holds r per cell; a stamped cell's r
+// Layout: the run's tree lives in LDS in place, node i at F[i - 1] (each
+// index holds its highest node, as the reference's array does).  Each thread
+// holds 4 consecutive cells and forms the nodes of 2 and 4 cells in
+// registers; a wavefront covers 256 cells and forms the nodes of 8..256 cells
+// by lane shuffles (the lane whose quad ends the block adds the left half
+// from the lane below), so only nodes of 512 cells and more need LDS
+// barriers.  Every node is (left child) + (right child) in fp32, as in a
+// level-by-level build, and a node exists only when its whole block lies in
+// the run.  The descent's node addresses depend only on x, so all its loads
+// are issued together; unset bits add +0.0, an exact identity on the
+// non-negative sums.
+//
+// FLT: grids with fp32 capacities.  remf holds r per cell; a marked cell's r
 // is lowered by 1.0f in fp32 (bit_util.hpp:87, d_cube_data_bit[tid]--) and the
 // integer rem counts the commits; the cell cost is cost_f32(model, r).
 template <bool FLT>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
-                                                 const int32_t* __restrict__ stamp, float* __restrict__ pfx,
+                                                 uint8_t* __restrict__ mark, float* __restrict__ pfx,
                                                  const float* __restrict__ lut,
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
@@ -47,10 +60,12 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
   extern __shared__ float sh[];
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
-  float* lv = sh + kLut;
-  __shared__ int offk[16];
-  for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
-  __syncthreads();
+  float* F = sh + kLut;
+  if (!FLT) {
+    for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
   const int nr = g.Z * M;
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
     const int z = r / M, line = r - z * M;
@@ -59,42 +74,70 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     if (!all && run_stamp[r] != bstamp) continue;
     const int L = (z & 1) ? g.Y : g.X;
     const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-      if constexpr (FLT) {
-        float r = remf[base + i];
-        if (!all && stamp[base + i] == bstamp) {
-          r -= 1.0f;
-          remf[base + i] = r;
-          rem[base + i] -= 1;
+    for (int c0 = 0; c0 < L; c0 += 1024) {
+      const int t4 = c0 + 4 * (int)threadIdx.x;
+      float v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = t4 + q;
+        float c = 0.0f;
+        if (i < L) {
+          const bool hit = !all && mark[base + i];
+          if constexpr (FLT) {
+            float rf = remf[base + i];
+            if (hit) {
+              rf -= 1.0f;
+              remf[base + i] = rf;
+              rem[base + i] -= 1;
+            }
+            c = cost_f32(model, rf);
+          } else {
+            int vv = rem[base + i];
+            if (hit) {
+              vv -= 1;
+              rem[base + i] = vv;
+            }
+            vv = vv < kRMin ? kRMin : (vv > kRMin + kLut - 1 ? kRMin + kLut - 1 : vv);
+            c = slut[vv - kRMin];
+          }
+          if (hit) mark[base + i] = 0;
         }
-        lv[i] = cost_f32(model, r);
-      } else {
-        int v = rem[base + i];
-        if (!all && stamp[base + i] == bstamp) {
-          v -= 1;
-          rem[base + i] = v;
-        }
-        v = v < kRMin ? kRMin : (v > kRMin + kLut - 1 ? kRMin + kLut - 1 : v);
-        lv[i] = slut[v - kRMin];
+        v[q] = c;
       }
-    }
-    if (threadIdx.x == 0) {
-      int off = 0;
-      for (int k = 0; (L >> k) > 0; ++k) { offk[k] = off; off += L >> k; }
+      // F[t4] = node(t4+1) = cell; F[t4+1] = node(t4+2) = 2 cells; F[t4+2] =
+      // cell; F[t4+3] = node(t4+4), 4 cells or more: built up below
+      const float p0 = v[0] + v[1], p1 = v[2] + v[3];
+      float q = p0 + p1;
+      float top = q;
+#pragma unroll
+      for (int k = 3; k <= 8; ++k) {
+        const int s2 = 1 << (k - 3);  // lanes per half block
+        const float left = __shfl_up(q, s2, 64);
+        q = left + q;
+        // this lane ends a block of 2^k cells: node(t4 + 4) gets it
+        if (((lane + 1) & ((2 << (k - 3)) - 1)) == 0) top = q;
+      }
+      if (t4 < L) F[t4] = v[0];
+      if (t4 + 1 < L) F[t4 + 1] = p0;
+      if (t4 + 2 < L) F[t4 + 2] = v[2];
+      if (t4 + 3 < L) F[t4 + 3] = top;
     }
     __syncthreads();
-    for (int k = 1; (L >> k) > 0; ++k) {
-      const int nk = L >> k, o = offk[k], op = offk[k - 1];
-      for (int j = threadIdx.x; j < nk; j += blockDim.x) lv[o + j] = lv[op + 2 * j] + lv[op + 2 * j + 1];
+    // nodes of 512 cells and more: F[i-1] = F[i - 2^(k-1) - 1] + F[i-1],
+    // in place, one level per barrier
+    for (int k = 9; (1 << k) <= L; ++k) {
+      const int h = 1 << (k - 1);
+      for (int i = (threadIdx.x + 1) << k; i <= L; i += (int)blockDim.x << k) F[i - 1] = F[i - 1 - h] + F[i - 1];
       __syncthreads();
     }
     for (int x = threadIdx.x; x < L; x += blockDim.x) {
-      uint32_t i = (uint32_t)x + 1;
+      const uint32_t i1 = (uint32_t)x + 1;
       float res = 0.0f;
-      while (i) {
-        const int k = __builtin_ctz(i);
-        res += lv[offk[k] + (int)(i >> k) - 1];
-        i &= i - 1;
+#pragma unroll
+      for (int b = 0; b < 14; ++b) {
+        const uint32_t ib = (i1 >> b) << b;
+        const float nv = ((i1 >> b) & 1u) ? F[ib - 1] : 0.0f;
+        res += nv;
       }
       pfx[base + x] = res;
     }
@@ -262,7 +305,7 @@ struct TreeArgs {
   int64_t* out_edges;
   int32_t* out_lpat;
   float* out_cost;
-  int32_t* stamp;
+  uint8_t* mark;
   int32_t* run_stamp;
   const int32_t* ctr;  // device batch counter; its value is the batch stamp
   int32_t bstamp;      // filled from *ctr at kernel entry
@@ -282,13 +325,14 @@ __device__ __forceinline__ int2 recorded_edge(const TreeArgs& A, const int* root
   return root[j] == r ? make_int2(j, k) : make_int2(k, j);
 }
 
-// Commit marks are idempotent plain stores: the cell's stamp and its run's
-// stamp.  The demand update itself happens once per stamped cell in k_refresh.
+// Commit marks are idempotent plain stores: the cell's mark byte and its
+// run's stamp.  The demand update itself happens once per marked cell in
+// k_refresh, which also clears the mark.
 __device__ __forceinline__ void mark_run(const TreeArgs& A, int z, int line) {
   A.run_stamp[z * A.M + line] = A.bstamp;
 }
 __device__ __forceinline__ void mark_any(const TreeArgs& A, uint32_t cell) {
-  A.stamp[cell] = A.bstamp;
+  A.mark[cell] = 1;
   int z, x, y;
   dec(A.g, cell, z, x, y);
   mark_run(A, z, (z & 1) ? x : y);
@@ -296,13 +340,13 @@ __device__ __forceinline__ void mark_any(const TreeArgs& A, uint32_t cell) {
 __device__ __forceinline__ void mark_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
   const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
   const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
-  for (int t = lo; t <= hi; ++t) A.stamp[base + t] = A.bstamp;
+  for (int t = lo; t <= hi; ++t) A.mark[base + t] = 1;
   mark_run(A, z, line);
 }
 __device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int x, int y) {
   const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
   for (int l = lo + 1; l < hi; ++l) {
-    A.stamp[enc(A.g, l, x, y)] = A.bstamp;
+    A.mark[enc(A.g, l, x, y)] = 1;
     mark_run(A, l, (l & 1) ? x : y);
   }
 }

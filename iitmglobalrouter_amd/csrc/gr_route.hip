@@ -71,16 +71,16 @@ struct KTimer {
   }
 };
 
-size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
+size_t refresh_lds(const gr_ctx* c) { return (kLut + (size_t)c->M) * sizeof(float); }
 int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
 
 void launch_refresh(gr_ctx* c, int all) {
   if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
   else
     hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_stamp, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -240,7 +240,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.out_edges = s.d_edges;
   A.out_lpat = s.d_lpat;
   A.out_cost = s.d_cost;
-  A.stamp = c->d_stamp;
+  A.mark = c->d_mark;
   A.run_stamp = c->d_run_stamp;
   A.ctr = c->d_ctr;
   A.bstamp = 0;
@@ -357,7 +357,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
     return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
-  if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_stamp, cs)) ||
+  if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
       (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
       (capf && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs))))) {
@@ -384,7 +384,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipFuncSetAttribute((const void*)k_tree_large, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
-      hipMemset(c->d_stamp, 0, sizeof(int32_t) * cs) != hipSuccess ||
+      hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
     g_err = "device set-up failed";
@@ -414,7 +414,7 @@ void gr_destroy(gr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_stamp); (void)hipFree(c->d_run_stamp);
+  (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_mark); (void)hipFree(c->d_run_stamp);
   (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
   (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
