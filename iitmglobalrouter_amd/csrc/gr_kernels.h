@@ -8,6 +8,8 @@
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
+//   vterm[cell]      fp32   Q(pos) - Q(pos-1): the via cost's single-cell term
+//                           (bit_util.hpp:108-117), one load instead of two
 //   mark[cell]       uint8  1 from the commit that used the cell until the
 //                           same batch's refresh consumes it
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
@@ -56,11 +58,13 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const float* __restrict__ lut,
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
-                                                 float* __restrict__ remf, int model) {
+                                                 float* __restrict__ remf, int model,
+                                                 float* __restrict__ vterm) {
   extern __shared__ float sh[];
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* F = sh + kLut;
+  float* R = F + M;  // the run's prefix, for the single-cell terms
   if (!FLT) {
     for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
     __syncthreads();
@@ -140,7 +144,12 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
         res += nv;
       }
       pfx[base + x] = res;
+      R[x] = res;
     }
+    __syncthreads();
+    // single-cell term of the reference's via cost, Q(pos) - Q(pos - 1) with
+    // Q(-1) = 0 (bit_util.hpp:108-117), the same fp32 subtraction
+    for (int x = threadIdx.x; x < L; x += blockDim.x) vterm[base + x] = R[x] - (x ? R[x - 1] : 0.0f);
     __syncthreads();
   }
 }
@@ -160,6 +169,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // strict-less in ascending p, i.e. the lexicographic minimum of (cost, p).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
+                                                   const float* __restrict__ vterm,
                                                    const int32_t* __restrict__ pins,
                                                    const int64_t* __restrict__ pinofs,
                                                    const int64_t* __restrict__ bpair,
@@ -183,8 +193,8 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
 #pragma unroll
   for (int i = 0; i < ZT - 1; ++i) {
-    ds[i] = dterm(pfx, g, i, x1, y1);
-    dd[i] = dterm(pfx, g, i, x2, y2);
+    ds[i] = vterm[enc(g, i, x1, y1)];
+    dd[i] = vterm[enc(g, i, x2, y2)];
   }
 #pragma unroll
   for (int l = 1; l < ZT; ++l) {
@@ -194,8 +204,8 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   if (D) {
 #pragma unroll
     for (int i = 0; i < ZT - 1; ++i) {
-      dc1[i] = dterm(pfx, g, i, x2, y1);  // corner after a horizontal first wire
-      dc2[i] = dterm(pfx, g, i, x1, y2);  // corner after a vertical first wire
+      dc1[i] = vterm[enc(g, i, x2, y1)];  // corner after a horizontal first wire
+      dc2[i] = vterm[enc(g, i, x1, y2)];  // corner after a vertical first wire
     }
 #pragma unroll
     for (int l = 1; l < ZT; ++l) {

@@ -71,16 +71,18 @@ struct KTimer {
   }
 };
 
-size_t refresh_lds(const gr_ctx* c) { return (kLut + (size_t)c->M) * sizeof(float); }
+size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
 int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
 
 void launch_refresh(gr_ctx* c, int all) {
   if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
+                       c->d_vterm);
   else
     hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model);
+                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
+                       c->d_vterm);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -184,8 +186,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   return 0;
 }
 
-typedef void (*LpatFn)(Geo, const float*, const int32_t*, const int64_t*, const int64_t*, const int32_t*, int,
-                       int64_t, float*, int16_t*, int32_t*);
+typedef void (*LpatFn)(Geo, const float*, const float*, const int32_t*, const int64_t*, const int64_t*,
+                       const int32_t*, int, int64_t, float*, int16_t*, int32_t*);
 
 LpatFn lane_kernel(int Z) {
   switch (Z) {
@@ -215,7 +217,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
     if (fn) {
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + 255) / 256)), dim3(256), 0, c->stream, c->g, c->d_pfx,
-                         c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
+                         c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
                          c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     } else {
       hipLaunchKernelGGL(k_lpat_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
@@ -358,7 +360,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
     return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
-      (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
+      (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_vterm, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
       (capf && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs))))) {
     g_err = c->err;
@@ -415,7 +417,7 @@ void gr_destroy(gr_ctx* c) {
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_mark); (void)hipFree(c->d_run_stamp);
-  (void)hipFree(c->d_pfx); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
+  (void)hipFree(c->d_pfx); (void)hipFree(c->d_vterm); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
   (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
   (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
