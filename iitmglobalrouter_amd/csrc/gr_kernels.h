@@ -531,10 +531,14 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
         if (i < n) {
           ri = root[i];
           const uint32_t rowi = (uint32_t)i * (uint32_t)(i - 1) / 2;
+          // branch-free so the loads of successive k overlap: a pair inside
+          // the component (incl. k == i, whose index is clamped) gives ~0
+#pragma unroll 4
           for (int k = part; k < n; k += S) {
-            if (root[k] == ri) continue;  // same component, incl. k == i
-            const uint32_t o = (k < i) ? rowi + k : (uint32_t)k * (uint32_t)(k - 1) / 2 + i;
-            const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+            const int rk = root[k];
+            const uint32_t o = (k < i) ? rowi + k : (k == i ? 0u : (uint32_t)k * (uint32_t)(k - 1) / 2 + i);
+            const unsigned long long key =
+                rk == ri ? ~0ull : (((unsigned long long)ord_key(cost[o]) << 32) | o);
             m = key < m ? key : m;
           }
         }
@@ -656,10 +660,11 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
       const int ri = root[tid];
       const uint32_t rowi = (uint32_t)tid * (uint32_t)(tid - 1) / 2;
       unsigned long long m = ~0ull;
+#pragma unroll 4
       for (int k = 0; k < n; ++k) {
-        if (root[k] == ri) continue;
-        const uint32_t o = (k < tid) ? rowi + k : (uint32_t)k * (uint32_t)(k - 1) / 2 + tid;
-        const unsigned long long key = ((unsigned long long)ord_key(cost[o]) << 32) | o;
+        const int rk = root[k];
+        const uint32_t o = (k < tid) ? rowi + k : (k == tid ? 0u : (uint32_t)k * (uint32_t)(k - 1) / 2 + tid);
+        const unsigned long long key = rk == ri ? ~0ull : (((unsigned long long)ord_key(cost[o]) << 32) | o);
         m = key < m ? key : m;
       }
       if (m != ~0ull) atomicMin(&best[ri], m);
