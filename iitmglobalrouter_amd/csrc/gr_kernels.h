@@ -214,6 +214,27 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
     }
   }
 
+  // Destination via stack from the last wire layer zs (a compile-time value
+  // in the unrolled loops below) to z2: cells z2..zs-1 going down, zs..z2-1
+  // going up, added in ascending layer order.  ddn[i] = dd[i] for i >= z2 and
+  // dup[i] = dd[i] for i < z2, +0.0f elsewhere, so each pattern adds ddn[0..zs)
+  // and, only when z2 > zs, dup[zs..).  The running cost is never -0.0, so
+  // adding +0.0f leaves it unchanged and the sums equal the cell-by-cell ones.
+  float ddn[ZT - 1], dup[ZT - 1];
+#pragma unroll
+  for (int i = 0; i < ZT - 1; ++i) {
+    ddn[i] = i >= z2 ? dd[i] : 0.f;
+    dup[i] = i < z2 ? dd[i] : 0.f;
+  }
+  auto to_dst = [&](float& c, const int zs) {
+#pragma unroll
+    for (int i = 0; i < zs; ++i) c += ddn[i];
+    if (z2 > zs) {
+#pragma unroll
+      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
+    }
+  };
+
   uint64_t best = ~0ull;
   float vs = 0.f;  // running source via sum for a > z1
 #pragma unroll
@@ -229,31 +250,30 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
     }
     const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
     if (moved1) ca += w1[a];
+    if (D) {
+      // second wire on b' (b' = 0 leaves both coordinates to cover: invalid)
 #pragma unroll
-    for (int b = 0; b < Z2T; ++b) {
-      if (!D && b > 0) continue;
-      const int bp = (a & 1) ? 2 * b : 2 * b + 1;
-      float c = ca;
-      int zs = a;
-      if (D && bp != 0) {
-        if (bp >= ZT) continue;
+      for (int b = 0; b < Z2T; ++b) {
+        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+        if (bp == 0 || bp >= ZT) continue;
+        float c = ca;
         const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
 #pragma unroll
         for (int i = lo; i < hi; ++i) c += (a & 1) ? dc2[i] : dc1[i];
         c += w2[bp];
-        zs = bp;
-      } else {
-        const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
-        if (!at_dst) continue;
+        to_dst(c, bp);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
+        best = key < best ? key : best;
       }
-      if (zs != z2) {
-        const int lo = zs < z2 ? zs : z2, hi = zs < z2 ? z2 : zs;
-#pragma unroll
-        for (int i = 0; i < ZT - 1; ++i)
-          if (i >= lo && i < hi) c += dd[i];
+    } else {
+      // straight pair: every b gives the same route, b = 0 is the lowest p
+      const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
+      if (at_dst) {
+        float c = ca;
+        to_dst(c, a);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T);
+        best = key < best ? key : best;
       }
-      const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
-      best = key < best ? key : best;
     }
   }
   out_cost[w] = key_cost((uint32_t)(best >> 32));
