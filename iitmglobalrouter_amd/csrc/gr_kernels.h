@@ -443,6 +443,19 @@ __device__ void tree2_commit(const TreeArgs& A, int pos) {
   A.out_cost[A.cost_base + pos] = 0.0f + A.pair_cost[A.pair[pos]];
 }
 
+// Root of x with path halving: each visited non-root is re-pointed to its
+// grandparent, an ancestor, so concurrent finds and hookings (which only
+// change roots) stay valid and chains built by one round's unions shrink.
+__device__ __forceinline__ int dsu_find(volatile int* vp, int x) {
+  while (true) {
+    const int p = vp[x];
+    if (p == x) return x;
+    const int pp = vp[p];
+    if (pp != p) vp[x] = pp;
+    x = pp;
+  }
+}
+
 // Lock-free DSU hooking on LDS: the smaller root is always hooked under the
 // larger (as gmst_util.hpp:314-325 does serially), so every component's root
 // stays its largest member whatever the order of concurrent unions.  A failed
@@ -450,8 +463,8 @@ __device__ void tree2_commit(const TreeArgs& A, int pos) {
 __device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
   volatile int* vp = par;
   for (int guard = 0; guard < bound; ++guard) {
-    while (vp[a] != a) a = vp[a];
-    while (vp[b] != b) b = vp[b];
+    a = dsu_find(vp, a);
+    b = dsu_find(vp, b);
     if (a == b) return;
     if (a > b) { const int t = a; a = b; b = t; }
     if (atomicCAS(&par[a], a, b) == a) return;
@@ -475,34 +488,40 @@ __device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
 // Rounds repeat until deg-1 edges (the reference's feedback loop,
 // mst_util.hpp:298-314, ends in the same state).
 __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int* root, unsigned long long* best,
-                            int* partner, int* par, int2* edges, float* ecost, int* wcnt, int* sh, float* scost,
-                            int scap) {
+                            int* partner, int* par_a, int* par_b, int2* edges, float* ecost, int* wcnt, int* sh,
+                            float* scost, int scap) {
   const int64_t pb = A.pinofs[pos];
   const int n = A.deg[pos];
   const float* gcost = A.pair_cost + A.pair[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
   const uint32_t P = (uint32_t)((int64_t)n * (n - 1) / 2);
   const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
-  // every round reads all P costs: keep them in LDS when they fit
+  // every round reads all P costs: keep them in LDS when they fit, as the
+  // orderable keys the minimum compares (ord_key), converted once here
   const float* cost = gcost;
-  if (P <= (uint32_t)scap) {
-    for (uint32_t o = tid; o < P; o += gsize) scost[o] = gcost[o];
-    cost = scost;
+  const bool staged = P <= (uint32_t)scap;
+  uint32_t* skey = (uint32_t*)scost;
+  if (staged) {
+    for (uint32_t o = tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
   }
-  for (int i = tid; i < n; i += gsize) root[i] = i;
+  for (int i = tid; i < n; i += gsize) {
+    root[i] = i;
+    best[i] = ~0ull;
+    par_a[i] = i;
+  }
   if (tid == 0) sh[0] = 0;
   __syncthreads();
   // Each round adds at least one edge (the globally smallest key is a mutual
   // pick), so n rounds always suffice; the bound only guards termination.
+  // The DSU parents alternate between two arrays: the flatten of one round
+  // reads this round's while it prepares the next round's (and resets best,
+  // which nothing reads after the accept), saving a barrier per round.
   for (int round = 0; round < n; ++round) {
     const int ne = sh[0];
     if (ne >= n - 1) break;
-    for (int i = tid; i < n; i += gsize) {
-      best[i] = ~0ull;
-      par[i] = i;
-    }
-    __syncthreads();
-    if (cost == gcost) {
+    int* par = (round & 1) ? par_b : par_a;
+    int* par_next = (round & 1) ? par_a : par_b;
+    if (!staged) {
       // costs did not fit in LDS: each thread streams a contiguous chunk of
       // pairs from global memory.  The row root's running minimum and that of
       // the current run of equal column roots stay in registers and go to LDS
@@ -542,26 +561,42 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
     } else {
       // node-centric: S adjacent threads scan node i's n-1 pairs (rows and
       // columns of the triangle) into a register min; one atomic per node.
+      // A thread visits its pairs in increasing index (its row part, then
+      // its column part), so a strict-less 32-bit key compare keeps the
+      // lowest index among equal keys: the (key, index) minimum.  Column
+      // indices advance by additions only.
       const int S = (n * 8 <= gsize) ? 8 : (n * 4 <= gsize) ? 4 : (n * 2 <= gsize) ? 2 : 1;
+      const int lgS = S == 8 ? 3 : S == 4 ? 2 : S == 2 ? 1 : 0;
       const int per = gsize / S;
       for (int b0 = 0; b0 < n; b0 += per) {
         const int i = b0 + tid / S, part = tid & (S - 1);
-        unsigned long long m = ~0ull;
+        uint32_t mk = ~0u, mo = ~0u;
         int ri = -1;
         if (i < n) {
           ri = root[i];
-          const uint32_t rowi = (uint32_t)i * (uint32_t)(i - 1) / 2;
-          // branch-free so the loads of successive k overlap: a pair inside
-          // the component (incl. k == i, whose index is clamped) gives ~0
+          // row part: pairs (i, k), k < i, index i(i-1)/2 + k
+          uint32_t o = (uint32_t)i * (uint32_t)(i - 1) / 2 + (uint32_t)part;
 #pragma unroll 4
-          for (int k = part; k < n; k += S) {
-            const int rk = root[k];
-            const uint32_t o = (k < i) ? rowi + k : (k == i ? 0u : (uint32_t)k * (uint32_t)(k - 1) / 2 + i);
-            const unsigned long long key =
-                rk == ri ? ~0ull : (((unsigned long long)ord_key(cost[o]) << 32) | o);
-            m = key < m ? key : m;
+          for (int k = part; k < i; k += S, o += S) {
+            const uint32_t key = skey[o];
+            const bool take = (root[k] != ri) & (key < mk);
+            mk = take ? key : mk;
+            mo = take ? o : mo;
+          }
+          // column part: pairs (k, i), k > i, index k(k-1)/2 + i
+          int k = i + 1 + ((part - (i + 1)) & (S - 1));
+          o = (uint32_t)k * (uint32_t)(k - 1) / 2 + (uint32_t)i;
+          const uint32_t cS = (uint32_t)(S * (S - 1) / 2);
+#pragma unroll 4
+          for (; k < n; k += S) {
+            const uint32_t key = skey[o];
+            const bool take = (root[k] != ri) & (key < mk);
+            mk = take ? key : mk;
+            mo = take ? o : mo;
+            o += ((uint32_t)k << lgS) + cS;
           }
         }
+        unsigned long long m = mk == ~0u ? ~0ull : (((unsigned long long)mk << 32) | mo);
         for (int sh2 = 1; sh2 < S; sh2 <<= 1) {
           const unsigned long long t = shfl_xor_u64(m, sh2);
           m = t < m ? t : m;
@@ -598,7 +633,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
         for (int q = 0; q < it * nwv + wv; ++q) off += wcnt[q];
         off += __popcll(m & ((1ull << lane) - 1ull));
         edges[off] = recorded_edge(A, root, r, pr, (uint32_t)best[r]);
-        ecost[off] = cost[(uint32_t)best[r]];
+        ecost[off] = key_cost((uint32_t)(best[r] >> 32));  // pair costs are never -0.0
         dsu_union(par, r, pr, 4 * n);
       }
     }
@@ -610,9 +645,9 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
     }
     // flatten: every node's root becomes its component's largest member
     for (int i = tid; i < n; i += gsize) {
-      int x = root[i];
-      while (par[x] != x) x = par[x];
-      root[i] = x;
+      root[i] = dsu_find(par, root[i]);
+      best[i] = ~0ull;
+      par_next[i] = i;
     }
     __syncthreads();
   }
@@ -708,11 +743,7 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     }
     ne += __popcll(gm);
     wave_sync();
-    if (me) {
-      int x = root[tid];
-      while (link[x] != x) x = link[x];
-      root[tid] = x;
-    }
+    if (me) root[tid] = dsu_find(link, root[tid]);
     wave_sync();
   }
   if (!valid) return;
@@ -778,12 +809,13 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int s
   int* root = (int*)(edges + cap);
   int* partner = root + cap;
   int* link = partner + cap;
-  float* ecost = (float*)(link + cap);
+  int* link2 = link + cap;
+  float* ecost = (float*)(link2 + cap);
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
   float* scost = ecost + cap;  // scap floats
-  tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, edges, ecost, wcnt,
-                     s_sh, scost, scap);
+  tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, link2, edges, ecost, wcnt,
+              s_sh, scost, scap);
 }
 
 }  // namespace grd

@@ -275,9 +275,9 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       A.list = s.d_list + m.cls_ofs[kClsLarge];
       A.n_list = m.cls_n[kClsLarge];
       const int cap = m.max_large;
-      const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 32) / 4;
+      const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
       const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
-      const size_t lds = (size_t)cap * 32 + (size_t)scap * 4;
+      const size_t lds = (size_t)cap * 36 + (size_t)scap * 4;
       hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(c->tune.large_threads), lds, lst, A, cap, scap);
     }
     if (fork) {
