@@ -675,7 +675,7 @@ struct GroupLds {
   int link[64];
   int2 edges[64];
   float ecost[64];
-  float cost[kSmallDeg * (kSmallDeg - 1) / 2];  // per group: G(G-1)/2 floats
+  uint32_t key[kSmallDeg * (kSmallDeg - 1) / 2];  // per group: G(G-1)/2 pair keys (ord_key of the cost)
 };
 
 template <int G>
@@ -694,10 +694,10 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
   int* link = L.link + gb;
   int2* edges = L.edges + gb;
   float* ecost = L.ecost + gb;
-  float* cost = L.cost + (gb / G) * (G * (G - 1) / 2);
+  uint32_t* key = L.key + (gb / G) * (G * (G - 1) / 2);
   if (valid) {
     const float* gcost = A.pair_cost + A.pair[pos];
-    for (uint32_t o = tid; o < P; o += G) cost[o] = gcost[o];
+    for (uint32_t o = tid; o < P; o += G) key[o] = ord_key(gcost[o]);
   }
   if (tid < n) root[tid] = tid;
   int ne = 0;
@@ -712,17 +712,28 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     }
     wave_sync();
     if (me) {
+      // pairs of node tid in increasing index (row, then column), strict-less
+      // on the 32-bit key: the (key, index) minimum, as in tree_commit
       const int ri = root[tid];
-      const uint32_t rowi = (uint32_t)tid * (uint32_t)(tid - 1) / 2;
-      unsigned long long m = ~0ull;
+      uint32_t mk = ~0u, mo = ~0u;
+      uint32_t o = (uint32_t)tid * (uint32_t)(tid - 1) / 2;
 #pragma unroll 4
-      for (int k = 0; k < n; ++k) {
-        const int rk = root[k];
-        const uint32_t o = (k < tid) ? rowi + k : (k == tid ? 0u : (uint32_t)k * (uint32_t)(k - 1) / 2 + tid);
-        const unsigned long long key = rk == ri ? ~0ull : (((unsigned long long)ord_key(cost[o]) << 32) | o);
-        m = key < m ? key : m;
+      for (int k = 0; k < tid; ++k, ++o) {
+        const uint32_t kv = key[o];
+        const bool take = (root[k] != ri) & (kv < mk);
+        mk = take ? kv : mk;
+        mo = take ? o : mo;
       }
-      if (m != ~0ull) atomicMin(&best[ri], m);
+      o = (uint32_t)(tid + 1) * (uint32_t)tid / 2 + (uint32_t)tid;
+#pragma unroll 4
+      for (int k = tid + 1; k < n; ++k) {
+        const uint32_t kv = key[o];
+        const bool take = (root[k] != ri) & (kv < mk);
+        mk = take ? kv : mk;
+        mo = take ? o : mo;
+        o += (uint32_t)k;
+      }
+      if (mk != ~0u) atomicMin(&best[ri], ((unsigned long long)mk << 32) | mo);
     }
     wave_sync();
     int pr = -1;
@@ -738,7 +749,7 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
     if (acc) {
       const int off = ne + __popcll(gm & ((1ull << lane) - 1ull));
       edges[off] = recorded_edge(A, root, tid, pr, (uint32_t)best[tid]);
-      ecost[off] = cost[(uint32_t)best[tid]];
+      ecost[off] = key_cost((uint32_t)(best[tid] >> 32));  // pair costs are never -0.0
       dsu_union(link, tid, pr, 4 * G);
     }
     ne += __popcll(gm);
