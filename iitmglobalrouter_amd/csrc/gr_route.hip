@@ -656,7 +656,7 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
   for (int gm : {64, 32})
     for (int lt : {1024, 512, 256})
       for (int fk : {1, 0})
-        for (int rb : {1024, 2048, 4096})
+        for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
             if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z
             gr_tuning t = {lt, gm, fk, rb, lw};
