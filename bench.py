@@ -129,6 +129,33 @@ def io_eval(gr, r, X, Y, Z, cap, pfx, pins, ofst, ids, cpu=True):
     return out
 
 
+def mst_side(gr, r, cap, n_nets, args):
+    """The pass with minimum spanning trees (GR_TREE_TRUE_MST): time and route
+    cost beside the root-pair pass's; the context is left in root-pair mode
+    holding a root-pair pass."""
+    rp_cost = gr.route_cost(cap, r.demand())
+    r.set_tree_mode(True)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset_demand()
+        r.route_schedule()
+    r.sync()
+    mst_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    mst_cost = gr.route_cost(cap, r.demand())
+    out = {"ms_per_step": round(mst_ms, 4), "nets_per_s": round(n_nets / (mst_ms / 1e3), 1),
+           "route_cost": mst_cost, "root_pair_route_cost": rp_cost,
+           "route_cost_change": round((mst_cost["total"] - rp_cost["total"]) / rp_cost["total"], 5)}
+    r.set_tree_mode(False)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    print(f"# true-MST trees {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
     """Learned proposal on the same generator: samples from exhaustive routing of
     seeds 1..train_seeds train the MLP on the GPU; seed 0 is then routed with the
@@ -207,6 +234,7 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, help="pick schedule parameters by timing (0: defaults)")
     ap.add_argument("--tuning", default="", help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave")
     ap.add_argument("--io", type=int, default=1, help="also measure evaluator, writer and reader (0: skip)")
+    ap.add_argument("--mst", type=int, default=1, help="also time true-MST trees for route quality (0: skip)")
     ap.add_argument("--train-seeds", type=int, default=20)
     ap.add_argument("--train-per-batch", type=int, default=1500)
     ap.add_argument("--train-epochs", type=int, default=4)
@@ -301,28 +329,10 @@ def main():
                 roof["traffic"] = json.load(f)["per_launch"][KNAMES[dom]]["traffic_bytes"]
     # side figure: the same pass with minimum spanning trees (GR_TREE_TRUE_MST)
     # instead of the reference's root-pair trees, for route quality
-    rp_cost = gr.route_cost(cap, r.demand())
-    r.set_tree_mode(True)
-    r.reset_demand()
-    r.route_schedule()
-    r.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.reset_demand()
-        r.route_schedule()
-    r.sync()
-    mst_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    mst_cost = gr.route_cost(cap, r.demand())
-    true_mst = {"ms_per_step": round(mst_ms, 4), "nets_per_s": round(n_nets / (mst_ms / 1e3), 1),
-                "route_cost": mst_cost, "root_pair_route_cost": rp_cost,
-                "route_cost_change": round((mst_cost["total"] - rp_cost["total"]) / rp_cost["total"], 5)}
-    r.set_tree_mode(False)
-    r.reset_demand()
-    r.route_schedule()
-    r.sync()
-    print(f"# true-MST trees {true_mst}", file=sys.stderr, flush=True)
+    true_mst = mst_side(gr, r, cap, n_nets, args) if args.mst else None
     io = None
     if args.io:
+
         io = io_eval(gr, r, X, Y, Z, cap, pfx, pins, ofst, ids, cpu=not args.no_cpu)
         print(f"# io {io}", file=sys.stderr, flush=True)
     learned = None
