@@ -187,6 +187,172 @@ __device__ __forceinline__ int pair_owner_wave(const int64_t* __restrict__ bpair
   return base + lo;
 }
 
+// Lexicographic minimum key (ord_key(cost) << 32 | p) over every pattern of
+// the pair (z1, x1, y1) -> (z2, x2, y2), for one lane; g.Z must equal ZT.
+// All via and wire terms of the pair are loaded once into registers; patterns
+// are then evaluated in ascending p with exactly the oracle's fp32 operation
+// sequence (bit_util.hpp:131-179):
+//   cost = 0; += via cells z1..a (ascending); += wire on a;
+//   [D and b' != 0: += via cells a..b' at the corner (ascending); += wire on b'];
+//   += via cells to z2 at the destination (ascending).
+// The source via sum for a > z1 is a running sum shared across a.  For
+// straight pairs (not D) all b tie, so only b = 0 is evaluated.  Selection is
+// strict-less in ascending p, i.e. the lexicographic minimum of (cost, p).
+//
+// SEL: only patterns whose first wire layer a has parity par count (the
+// caller takes the other parity in another lane), only candidates among them
+// (bit q = p - Z2 of mask, q < 64) unless full, and only the terms those
+// patterns use are loaded (a lane's skipped loads make no memory request).  A
+// straight pair's candidates (a, b) all cost what (a, 0) costs, so the lowest
+// candidate p of that a stands for them.
+template <int ZT>
+__device__ __forceinline__ constexpr uint64_t parity_patterns(int par) {
+  uint64_t m = 0;
+  for (int q = 0; q < (ZT - 1) * ((ZT + 1) / 2); ++q)
+    if (((q / ((ZT + 1) / 2) + 1) & 1) == par) m |= 1ull << q;
+  return m;
+}
+
+template <int ZT, bool SEL = false>
+__device__ __forceinline__ uint64_t lane_best(const Geo& g, const float* __restrict__ pfx,
+                                              const float* __restrict__ vterm, int z1, int x1, int y1, int z2,
+                                              int x2, int y2, uint64_t mask = ~0ull, int par = 0,
+                                              bool full = true) {
+  constexpr int Z2T = (ZT + 1) / 2;
+  static_assert(!SEL || (ZT - 1) * Z2T <= 64, "candidate masks hold 64 patterns");
+  const bool D = (x1 != x2) && (y1 != y2);
+  if (SEL) mask = (full ? ~0ull : mask) & (par ? parity_patterns<ZT>(1) : parity_patterns<ZT>(0));
+
+  // terms the patterns use: bit i of nds / ndd / ndc1 / ndc2 (via cells of
+  // layer i), bit l of nw1 / nw2 (wires on layer l)
+  uint32_t nds = ~0u, ndd = ~0u, ndc1 = ~0u, ndc2 = ~0u, nw1 = ~0u, nw2 = ~0u;
+  if (SEL && full) {  // every pattern of the parity
+    constexpr uint32_t odd = 0xAAAAAAAAu;
+    nw1 = par ? odd : ~odd;
+    nw2 = par ? ~odd : odd;
+    ndc1 = par ? 0u : ~0u;
+    ndc2 = par ? ~0u : 0u;
+  } else if (SEL) {
+    nds = ndd = ndc1 = ndc2 = nw1 = nw2 = 0;
+    auto rng = [](int u, int v) {  // bits [min, max)
+      const int lo = u < v ? u : v, hi = u < v ? v : u;
+      return ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+    };
+    for (uint64_t m = mask; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m), a = q / Z2T + 1, b = q - (a - 1) * Z2T;
+      int zl = a;
+      if (D) {
+        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+        if (bp == 0 || bp >= ZT) continue;
+        if (a & 1) ndc2 |= rng(a, bp); else ndc1 |= rng(a, bp);
+        nw2 |= 1u << bp;
+        zl = bp;
+      } else if ((a & 1) ? (x1 != x2) : (y1 != y2)) {
+        continue;  // the first wire does not reach the destination
+      }
+      nds |= rng(z1, a);
+      nw1 |= 1u << a;
+      ndd |= rng(zl, z2);
+    }
+  }
+
+  float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
+#pragma unroll
+  for (int i = 0; i < ZT - 1; ++i) {
+    ds[i] = ((nds >> i) & 1) ? vterm[enc(g, i, x1, y1)] : 0.f;
+    dd[i] = ((ndd >> i) & 1) ? vterm[enc(g, i, x2, y2)] : 0.f;
+  }
+#pragma unroll
+  for (int l = 1; l < ZT; ++l) {
+    // first wire on layer l from the source: rows y1 on even layers, columns x1 on odd
+    const bool moved = (l & 1) ? (y1 != y2) : (x1 != x2);
+    w1[l] = (moved && ((nw1 >> l) & 1)) ? ((l & 1) ? wterm(pfx, g, l, x1, y1, y2) : wterm(pfx, g, l, y1, x1, x2))
+                                        : 0.f;
+  }
+  if (D) {
+#pragma unroll
+    for (int i = 0; i < ZT - 1; ++i) {
+      dc1[i] = ((ndc1 >> i) & 1) ? vterm[enc(g, i, x2, y1)] : 0.f;  // corner after a horizontal first wire
+      dc2[i] = ((ndc2 >> i) & 1) ? vterm[enc(g, i, x1, y2)] : 0.f;  // corner after a vertical first wire
+    }
+#pragma unroll
+    for (int l = 1; l < ZT; ++l) {
+      // second wire into the destination: columns x2 on odd layers, rows y2 on even
+      w2[l] = ((nw2 >> l) & 1) ? ((l & 1) ? wterm(pfx, g, l, x2, y1, y2) : wterm(pfx, g, l, y2, x1, x2)) : 0.f;
+    }
+  }
+
+  // Destination via stack from the last wire layer zs (a compile-time value
+  // in the unrolled loops below) to z2: cells z2..zs-1 going down, zs..z2-1
+  // going up, added in ascending layer order.  ddn[i] = dd[i] for i >= z2 and
+  // dup[i] = dd[i] for i < z2, +0.0f elsewhere, so each pattern adds ddn[0..zs)
+  // and, only when z2 > zs, dup[zs..).  The running cost is never -0.0, so
+  // adding +0.0f leaves it unchanged and the sums equal the cell-by-cell ones.
+  float ddn[ZT - 1], dup[ZT - 1];
+#pragma unroll
+  for (int i = 0; i < ZT - 1; ++i) {
+    ddn[i] = i >= z2 ? dd[i] : 0.f;
+    dup[i] = i < z2 ? dd[i] : 0.f;
+  }
+  auto to_dst = [&](float& c, const int zs) {
+#pragma unroll
+    for (int i = 0; i < zs; ++i) c += ddn[i];
+    if (z2 > zs) {
+#pragma unroll
+      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
+    }
+  };
+
+  uint64_t best = ~0ull;
+  float vs = 0.f;  // running source via sum for a > z1
+#pragma unroll
+  for (int a = 1; a < ZT; ++a) {
+    float ca = 0.f;
+    if (a > z1) {
+      vs = (a == z1 + 1) ? ds[a - 1] : vs + ds[a - 1];
+      ca = vs;
+    } else if (a < z1) {
+#pragma unroll
+      for (int i = 0; i < ZT - 1; ++i)
+        if (i >= a && i < z1) ca += ds[i];
+    }
+    const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
+    if (moved1) ca += w1[a];
+    if (D) {
+      // second wire on b' (b' = 0 leaves both coordinates to cover: invalid)
+#pragma unroll
+      for (int b = 0; b < Z2T; ++b) {
+        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+        if (bp == 0 || bp >= ZT) continue;
+        if (SEL && !((mask >> ((a - 1) * Z2T + b)) & 1)) continue;  // also skips the other parity
+        float c = ca;
+        const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
+#pragma unroll
+        for (int i = lo; i < hi; ++i) c += (a & 1) ? dc2[i] : dc1[i];
+        c += w2[bp];
+        to_dst(c, bp);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
+        best = key < best ? key : best;
+      }
+    } else {
+      // straight pair: every b gives the same route, b = 0 is the lowest p
+      const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
+      int b0 = 0;
+      if (SEL) {
+        const uint64_t sub = (mask >> ((a - 1) * Z2T)) & ((1ull << Z2T) - 1);
+        b0 = sub ? __builtin_ctzll(sub) : -1;
+      }
+      if (at_dst && b0 >= 0) {
+        float c = ca;
+        to_dst(c, a);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b0);
+        best = key < best ? key : best;
+      }
+    }
+  }
+  return best;
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
   lo = __shfl_xor(lo, m, 64);

@@ -98,6 +98,8 @@ struct gr_ctx {
   // training-data capture after each batch's costing
   gr_mlp* collect = nullptr;
   int collect_per_batch = 0;
+  // every live model created on this context; gr_destroy detaches them
+  std::vector<gr_mlp*> models;
   bool timing = false;
   std::vector<grh::EvPair> evs;
   std::vector<hipEvent_t> ev_pool;
@@ -135,5 +137,13 @@ int dalloc(gr_ctx* c, T** p, size_t n) {
 // top-k patterns, and capture of (features, chosen pattern) training samples.
 int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& m);
 int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& m);
+// Called by gr_destroy: the context's models lose it (their device buffers are
+// still freed by gr_mlp_destroy; every other model call then fails).
+void mlp_detach_all(gr_ctx* c);
+// True when the model's costing is the single fused kernel (no allocation at
+// enqueue time, so a pass with it can be captured as a graph).
+bool mlp_fused(const gr_mlp* m);
+// Drops the captured whole-schedule graph (it holds the old costing set-up).
+void drop_graph(gr_ctx* c);
 
 }  // namespace grh

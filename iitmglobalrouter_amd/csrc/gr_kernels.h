@@ -158,15 +158,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // L-pattern costing
 // ---------------------------------------------------------------------------
 
-// One lane per pin pair.  All via and wire terms of the pair are loaded once
-// into registers; patterns are then evaluated in ascending p with exactly the
-// oracle's fp32 operation sequence (bit_util.hpp:131-179):
-//   cost = 0; += via cells z1..a (ascending); += wire on a;
-//   [D and b' != 0: += via cells a..b' at the corner (ascending); += wire on b'];
-//   += via cells to z2 at the destination (ascending).
-// The source via sum for a > z1 is a running sum shared across a.  For
-// straight pairs (not D) all b tie, so only b = 0 is evaluated.  Selection is
-// strict-less in ascending p, i.e. the lexicographic minimum of (cost, p).
+// One lane per pin pair of the batch.
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
                                                    const float* __restrict__ vterm,
@@ -176,7 +168,6 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
                                                    const int32_t* __restrict__ chunk_net, int n, int64_t P,
                                                    float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
                                                    int32_t* __restrict__ ctr) {
-  constexpr int Z2T = (ZT + 1) / 2;
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
   const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
@@ -188,94 +179,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
-  const bool D = (x1 != x2) && (y1 != y2);
-
-  float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
-#pragma unroll
-  for (int i = 0; i < ZT - 1; ++i) {
-    ds[i] = vterm[enc(g, i, x1, y1)];
-    dd[i] = vterm[enc(g, i, x2, y2)];
-  }
-#pragma unroll
-  for (int l = 1; l < ZT; ++l) {
-    // first wire on layer l from the source: rows y1 on even layers, columns x1 on odd
-    w1[l] = (l & 1) ? (y1 != y2 ? wterm(pfx, g, l, x1, y1, y2) : 0.f) : (x1 != x2 ? wterm(pfx, g, l, y1, x1, x2) : 0.f);
-  }
-  if (D) {
-#pragma unroll
-    for (int i = 0; i < ZT - 1; ++i) {
-      dc1[i] = vterm[enc(g, i, x2, y1)];  // corner after a horizontal first wire
-      dc2[i] = vterm[enc(g, i, x1, y2)];  // corner after a vertical first wire
-    }
-#pragma unroll
-    for (int l = 1; l < ZT; ++l) {
-      // second wire into the destination: columns x2 on odd layers, rows y2 on even
-      w2[l] = (l & 1) ? wterm(pfx, g, l, x2, y1, y2) : wterm(pfx, g, l, y2, x1, x2);
-    }
-  }
-
-  // Destination via stack from the last wire layer zs (a compile-time value
-  // in the unrolled loops below) to z2: cells z2..zs-1 going down, zs..z2-1
-  // going up, added in ascending layer order.  ddn[i] = dd[i] for i >= z2 and
-  // dup[i] = dd[i] for i < z2, +0.0f elsewhere, so each pattern adds ddn[0..zs)
-  // and, only when z2 > zs, dup[zs..).  The running cost is never -0.0, so
-  // adding +0.0f leaves it unchanged and the sums equal the cell-by-cell ones.
-  float ddn[ZT - 1], dup[ZT - 1];
-#pragma unroll
-  for (int i = 0; i < ZT - 1; ++i) {
-    ddn[i] = i >= z2 ? dd[i] : 0.f;
-    dup[i] = i < z2 ? dd[i] : 0.f;
-  }
-  auto to_dst = [&](float& c, const int zs) {
-#pragma unroll
-    for (int i = 0; i < zs; ++i) c += ddn[i];
-    if (z2 > zs) {
-#pragma unroll
-      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
-    }
-  };
-
-  uint64_t best = ~0ull;
-  float vs = 0.f;  // running source via sum for a > z1
-#pragma unroll
-  for (int a = 1; a < ZT; ++a) {
-    float ca = 0.f;
-    if (a > z1) {
-      vs = (a == z1 + 1) ? ds[a - 1] : vs + ds[a - 1];
-      ca = vs;
-    } else if (a < z1) {
-#pragma unroll
-      for (int i = 0; i < ZT - 1; ++i)
-        if (i >= a && i < z1) ca += ds[i];
-    }
-    const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
-    if (moved1) ca += w1[a];
-    if (D) {
-      // second wire on b' (b' = 0 leaves both coordinates to cover: invalid)
-#pragma unroll
-      for (int b = 0; b < Z2T; ++b) {
-        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
-        if (bp == 0 || bp >= ZT) continue;
-        float c = ca;
-        const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
-#pragma unroll
-        for (int i = lo; i < hi; ++i) c += (a & 1) ? dc2[i] : dc1[i];
-        c += w2[bp];
-        to_dst(c, bp);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
-        best = key < best ? key : best;
-      }
-    } else {
-      // straight pair: every b gives the same route, b = 0 is the lowest p
-      const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
-      if (at_dst) {
-        float c = ca;
-        to_dst(c, a);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T);
-        best = key < best ? key : best;
-      }
-    }
-  }
+  const uint64_t best = lane_best<ZT>(g, pfx, vterm, z1, x1, y1, z2, x2, y2);
   out_cost[w] = key_cost((uint32_t)(best >> 32));
   out_pat[w] = (int16_t)(uint32_t)best;
 }

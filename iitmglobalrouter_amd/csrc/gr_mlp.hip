@@ -346,6 +346,366 @@ __global__ __launch_bounds__(256) void k_lpat_topk(Geo g, const float* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused proposal: features -> three MFMA layers -> top-k costing, per tile
+// ---------------------------------------------------------------------------
+
+// A workgroup of 256 threads takes TT (64 or 128) consecutive pairs (or input
+// rows).  Activations stay in LDS as one [TT][RS] bf16 buffer (features, then
+// H1, then H2), so only the grid cells, the weights and the results cross the
+// memory hierarchy.  Layers run transposed: for neurons n
+// and pairs r, D[n][r] = sum_k W^T[n][k] X[r][k] with W^T in MFMA fragment
+// order (see k_pack) as the A operand from global memory (L2-resident, 280 KB
+// at the 256-wide model) and X from LDS as the B operand, so each lane's four
+// results are four consecutive neurons of one pair: one 8-byte LDS store.
+// Hidden layers: wave w owns neurons [64w, 64w + 64) for all TT pairs
+// (4 x TT/16 fragments).  Output layer: wave w owns pairs [TT/4 w, TT/4 (w + 1))
+// for all output neurons.  The K order of every sum and
+// the bias, ReLU and bf16 rounding points are those of k_gemm's forward pass.
+constexpr int kHid = 256;  // hidden width served by the fused kernel
+constexpr int kTile = 64;  // rows per workgroup of the fused kernel (four fit a CU)
+
+template <int ZT>
+struct PropShape {
+  static constexpr int NFEAT = 25 * ZT;                         // lattice features
+  static constexpr int KIN = (4 + NFEAT + 31) / 32 * 32;         // padded input width
+  static constexpr int RS = (KIN > kHid ? KIN : kHid) + 8;       // LDS row stride (bf16), +16 B against bank conflicts
+  static constexpr int NOUT = (ZT - 1) * ((ZT + 1) / 2);        // candidate patterns
+  static constexpr int NF3 = (NOUT + 15) / 16;                  // output fragments routing ranks
+  static constexpr int NFO = (NOUT + 63) / 64 * 4;              // all padded output columns (forward pass)
+};
+
+// Weight [K x N] (fp32, row-major) -> bf16 in MFMA A-fragment order: block
+// (f, s) of 64 lanes x 8 values holds W^T rows f*16 + (l % 16), columns
+// s*32 + 8*(l / 16) + e; rows >= N are zero.  One fragment load is then one
+// contiguous 1 KB read.
+__global__ __launch_bounds__(256) void k_pack(int K, int N, int NF, const float* __restrict__ W,
+                                              uint16_t* __restrict__ Wp) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int KS = K / 32;
+  if (i >= (int64_t)NF * KS * 512) return;
+  const int e = (int)(i & 7), l = (int)((i >> 3) & 63);
+  const int fs = (int)(i >> 9), s = fs % KS, f = fs / KS;
+  const int n = f * 16 + (l & 15), k = s * 32 + 8 * (l >> 4) + e;
+  Wp[i] = n < N ? f2bf(W[(int64_t)k * N + n]) : (uint16_t)0;
+}
+
+// acc[f][q] = sum over the K steps of W^T fragment (f, s) x X rows
+// prow0 + 16q .. + 16.
+template <int KS, int NF, int PF>
+__device__ __forceinline__ void mma_tile(const uint16_t* __restrict__ wp, const uint16_t* act, int RS, int prow0,
+                                         v4f (&acc)[NF][PF], int lane) {
+  const uint4* wv = reinterpret_cast<const uint4*>(wp) + lane;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int q = 0; q < PF; ++q) acc[f][q] = (v4f){0.f, 0.f, 0.f, 0.f};
+  const uint16_t* bp = act + (prow0 + (lane & 15)) * RS + 8 * (lane >> 4);
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    uint4 a[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) a[f] = wv[(f * KS + s) * 64];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const v8bf b = __builtin_bit_cast(v8bf, *(const uint4*)(bp + q * 16 * RS + s * 32));
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        acc[f][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a[f]), b, acc[f][q], 0, 0, 0);
+    }
+  }
+}
+
+// Hidden-layer epilogue: + bias, ReLU, bf16, into act[pair][neuron].
+template <int NF, int PF>
+__device__ __forceinline__ void store_hidden(const v4f (&acc)[NF][PF], const float* __restrict__ bias, int n0,
+                                             uint16_t* act, int RS, int lane) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int n = n0 + 16 * f + 4 * (lane >> 4);
+    const float4 bb = *reinterpret_cast<const float4*>(bias + n);
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      float v0 = acc[f][q][0] + bb.x, v1 = acc[f][q][1] + bb.y, v2 = acc[f][q][2] + bb.z, v3 = acc[f][q][3] + bb.w;
+      v0 = v0 > 0.f ? v0 : 0.f;
+      v1 = v1 > 0.f ? v1 : 0.f;
+      v2 = v2 > 0.f ? v2 : 0.f;
+      v3 = v3 > 0.f ? v3 : 0.f;
+      uint2 pk;
+      pk.x = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+      pk.y = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
+      *reinterpret_cast<uint2*>(act + (16 * q + (lane & 15)) * RS + n) = pk;
+    }
+  }
+}
+
+struct PropArgs {
+  Geo g;
+  const int32_t* rem;
+  const float* loglut;
+  const float* pfx;
+  const float* vterm;
+  const int32_t* pins;
+  const int64_t* pinofs;
+  const int64_t* bpair;
+  const int32_t* chunk_net;
+  int n;
+  int64_t P;          // pairs of the batch, or input rows
+  const uint16_t* X;  // input rows [P][k_in] (forward only)
+  const uint16_t *W1p, *W2p, *W3p;
+  const float *b1, *b2, *b3;
+  int n_op;
+  float* logits;  // [P][n_op] (forward only)
+  int topk;
+  float thr;
+  float* out_cost;
+  int16_t* out_pat;
+  int32_t* ctr;
+  unsigned long long* stats;
+};
+
+// ROUTE: features of the batch's pairs gathered from the grid, top-k selection
+// from the output fragments, then candidate costing with the full-enumeration
+// fallback (k_lpat_topk's rules).  !ROUTE: rows of A.X in, logits out (the
+// inference forward pass).
+template <int ZT, bool ROUTE, int TT>
+__global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
+  static_assert(TT == 64 || TT == 128, "tile of 64 or 128 rows");
+  constexpr int QW = TT / 64;  // output-layer pair fragments per wave
+  using S = PropShape<ZT>;
+  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, NF3 = S::NF3, NOUT = S::NOUT;
+  __shared__ __attribute__((aligned(16))) uint16_t act[TT * RS];
+  __shared__ uint16_t slut[ROUTE ? kLut : 1];         // log-cost table as bf16
+  __shared__ int16_t sx[ROUTE ? TT : 1][5], sy[ROUTE ? TT : 1][5];  // lattice columns / rows
+  __shared__ int16_t sc[ROUTE ? TT : 1][6];           // z1 x1 y1 z2 x2 y2
+  __shared__ unsigned long long smask[ROUTE ? TT : 1];  // candidate patterns q = p - Z2
+  __shared__ int8_t sfb[ROUTE ? TT : 1];              // low confidence: enumerate every pattern
+  __shared__ int nfb;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int64_t t0 = (int64_t)blockIdx.x * TT;
+  const int nrow = (int)(A.P - t0 < TT ? A.P - t0 : TT);
+  const Geo& g = A.g;
+
+  if (ROUTE) {
+    if (blockIdx.x == 0 && tid == 0) *A.ctr += 1;  // this batch's stamp
+    if (tid == 0) nfb = 0;
+    for (int i = tid; i < kLut; i += 256) slut[i] = f2bf(A.loglut[i]);
+    if (tid < TT) {  // waves 0 and 1 whole: the owner search is wave-collective
+      const int64_t w = t0 + (tid < nrow ? tid : nrow - 1);
+      const int pos = pair_owner_wave(A.bpair, A.chunk_net, A.n, w);
+      int j, k;
+      pair_decode((uint32_t)(w - A.bpair[pos]), j, k);
+      const int64_t pb = A.pinofs[pos];
+      int z1, x1, y1, z2, x2, y2;
+      dec(g, (uint32_t)A.pins[pb + j], z1, x1, y1);
+      dec(g, (uint32_t)A.pins[pb + k], z2, x2, y2);
+      sc[tid][0] = (int16_t)z1; sc[tid][1] = (int16_t)x1; sc[tid][2] = (int16_t)y1;
+      sc[tid][3] = (int16_t)z2; sc[tid][4] = (int16_t)x2; sc[tid][5] = (int16_t)y2;
+      const int dx = x2 - x1, dy = y2 - y1;
+#pragma unroll
+      for (int u = 0; u < 5; ++u) {
+        sx[tid][u] = (int16_t)(x1 + (dx * u + (dx >= 0 ? 2 : -2)) / 4);
+        sy[tid][u] = (int16_t)(y1 + (dy * u + (dy >= 0 ? 2 : -2)) / 4);
+      }
+      uint16_t* row = act + tid * RS;
+      row[0] = f2bf((float)dx * (1.0f / 64.0f));
+      row[1] = f2bf((float)dy * (1.0f / 64.0f));
+      row[2] = f2bf((float)z1);
+      row[3] = f2bf((float)z2);
+      for (int f = 4 + NFEAT; f < KIN; ++f) row[f] = 0;
+    }
+    __syncthreads();
+    // lattice features: one lattice point (pair, u, v) per thread and step,
+    // all ZT layers of it (feature 4 + l*25 + u*5 + v), UG points in flight
+    constexpr int NG = TT * 25, UG = 7;
+    for (int gb = tid; gb < NG; gb += 256 * UG) {
+      uint32_t be[UG], bo[UG];
+      int dst[UG];
+#pragma unroll
+      for (int u = 0; u < UG; ++u) {
+        const int gi0 = gb + 256 * u, gi = gi0 < NG ? gi0 : NG - 1;
+        const int pr = gi / 25, uv = gi - 25 * pr, iu = uv / 5, iv = uv - 5 * iu;
+        const uint32_t xs = (uint32_t)sx[pr][iu], ys = (uint32_t)sy[pr][iv];
+        be[u] = ys * (uint32_t)g.X + xs;  // even layers: rows
+        bo[u] = xs * (uint32_t)g.Y + ys;  // odd layers: columns
+        dst[u] = gi0 < NG ? pr * RS + 4 + uv : -1;
+      }
+      int32_t r[UG][ZT];
+#pragma unroll
+      for (int u = 0; u < UG; ++u)
+#pragma unroll
+        for (int l = 0; l < ZT; ++l) r[u][l] = A.rem[(uint32_t)l * g.XY + ((l & 1) ? bo[u] : be[u])];
+#pragma unroll
+      for (int u = 0; u < UG; ++u) {
+        if (dst[u] < 0) continue;
+#pragma unroll
+        for (int l = 0; l < ZT; ++l) {
+          const int v = r[u][l] < kRMin ? kRMin : (r[u][l] > kRMin + kLut - 1 ? kRMin + kLut - 1 : r[u][l]);
+          act[dst[u] + 25 * l] = slut[v - kRMin];
+        }
+      }
+    }
+  } else {
+    constexpr int C8 = KIN / 8;
+    for (int i = tid; i < TT * C8; i += 256) {
+      const int pr = i / C8, c = i - pr * C8;
+      const int64_t row = t0 + (pr < nrow ? pr : nrow - 1);
+      *reinterpret_cast<uint4*>(act + pr * RS + c * 8) =
+          *reinterpret_cast<const uint4*>(A.X + row * KIN + c * 8);
+    }
+  }
+  __syncthreads();
+
+  // hidden layers
+  {
+    v4f acc[4][TT / 16];
+    mma_tile<KIN / 32, 4, TT / 16>(A.W1p + (size_t)wv * 4 * (KIN / 32) * 512, act, RS, 0, acc, lane);
+    __syncthreads();
+    store_hidden<4, TT / 16>(acc, A.b1, wv * 64, act, RS, lane);
+    __syncthreads();
+    mma_tile<kHid / 32, 4, TT / 16>(A.W2p + (size_t)wv * 4 * (kHid / 32) * 512, act, RS, 0, acc, lane);
+    __syncthreads();
+    store_hidden<4, TT / 16>(acc, A.b2, wv * 64, act, RS, lane);
+    __syncthreads();
+  }
+
+  // output layer: lane l holds, for pair 32wv + 16q + (l & 15), the logits of
+  // output neurons 16f + 4(l >> 4) + i (f < NFL, i < 4)
+  constexpr int NFL = ROUTE ? NF3 : S::NFO;
+  v4f acc[NFL][QW];
+  mma_tile<kHid / 32, NFL, QW>(A.W3p, act, RS, 16 * QW * wv, acc, lane);
+  if (!ROUTE) {
+#pragma unroll
+    for (int f = 0; f < NFL; ++f) {
+      const int n = 16 * f + 4 * (lane >> 4);
+      const float4 bb = *reinterpret_cast<const float4*>(A.b3 + n);
+#pragma unroll
+      for (int q = 0; q < QW; ++q) {
+        const int pr = 16 * QW * wv + 16 * q + (lane & 15);
+        if (pr < nrow)
+          *reinterpret_cast<float4*>(A.logits + (t0 + pr) * A.n_op + n) =
+              make_float4(acc[f][q][0] + bb.x, acc[f][q][1] + bb.y, acc[f][q][2] + bb.z, acc[f][q][3] + bb.w);
+      }
+    }
+    return;
+  }
+
+  // top-k per pair over its 4 lanes: k rounds, each taking the best (value
+  // desc, index asc) logit not yet taken; softmax top-1 probability from the
+  // 4-lane max and sum
+  const int gq = lane >> 4;
+#pragma unroll
+  for (int q = 0; q < QW; ++q) {
+    float vals[NF3][4];
+    float mxl = -INFINITY;
+#pragma unroll
+    for (int f = 0; f < NF3; ++f) {
+      const int nb = 16 * f + 4 * gq;
+      const float4 bb = *reinterpret_cast<const float4*>(A.b3 + nb);
+      vals[f][0] = acc[f][q][0] + bb.x;
+      vals[f][1] = acc[f][q][1] + bb.y;
+      vals[f][2] = acc[f][q][2] + bb.z;
+      vals[f][3] = acc[f][q][3] + bb.w;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (nb + i < NOUT) mxl = fmaxf(mxl, vals[f][i]);
+    }
+    float mx = fmaxf(mxl, __shfl_xor(mxl, 16, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    float sl = 0.f;
+#pragma unroll
+    for (int f = 0; f < NF3; ++f)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (16 * f + 4 * gq + i < NOUT) sl += expf(vals[f][i] - mx);
+    float sm = sl + __shfl_xor(sl, 16, 64);
+    sm = sm + __shfl_xor(sm, 32, 64);
+    uint32_t taken = 0;  // bit f*4 + i: this lane's logit already chosen
+    uint64_t msk = 0;
+    for (int t = 0; t < A.topk; ++t) {
+      float bv = -INFINITY;
+      int bi = 127;  // none
+#pragma unroll
+      for (int f = 0; f < NF3; ++f)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int n = 16 * f + 4 * gq + i;
+          // ascending n, strict >: the lowest index wins among equal values
+          if (n < NOUT && !((taken >> (f * 4 + i)) & 1) && (vals[f][i] > bv || bi == 127)) {
+            bv = vals[f][i];
+            bi = n;
+          }
+        }
+#pragma unroll
+      for (int d = 16; d <= 32; d <<= 1) {
+        const float ov = __shfl_xor(bv, d, 64);
+        const int oi = __shfl_xor(bi, d, 64);
+        if (oi != 127 && (bi == 127 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+      }
+      if (bi < NOUT) {
+        msk |= 1ull << bi;
+        if (((bi >> 2) & 3) == gq) taken |= 1u << ((bi >> 4) * 4 + (bi & 3));  // this lane's logit
+      }
+    }
+    if (gq == 0) {
+      const int pr = 16 * QW * wv + 16 * q + (lane & 15);
+      smask[pr] = msk;
+      sfb[pr] = (int8_t)((1.0f / sm) < A.thr);
+    }
+  }
+  __syncthreads();
+
+  // candidate costing: waves 2h and 2h + 1 take pairs 64h .. 64h + 63, one
+  // parity of the first wire layer each (uniform per wave, so the loads do not
+  // diverge by parity); only the terms of the lane's candidates are loaded.
+  // The two keys meet in LDS; no valid candidate -> every pattern.
+  unsigned long long* skey = reinterpret_cast<unsigned long long*>(act);  // [2][TT]
+  if (64 * (wv >> 1) < TT) {
+    const int pr = 64 * (wv >> 1) + lane, par = wv & 1;
+    const int pc = pr < nrow ? pr : nrow - 1;
+    const int z1 = sc[pc][0], x1 = sc[pc][1], y1 = sc[pc][2], z2 = sc[pc][3], x2 = sc[pc][4], y2 = sc[pc][5];
+    skey[par * TT + pr] =
+        lane_best<ZT, true>(g, A.pfx, A.vterm, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0);
+  }
+  __syncthreads();
+  if (tid < nrow) {
+    const uint64_t k0 = skey[tid], k1 = skey[TT + tid];
+    uint64_t best = k0 < k1 ? k0 : k1;
+    bool fb = sfb[tid] != 0;
+    if (best == ~0ull) {
+      fb = true;
+      best = lane_best<ZT>(g, A.pfx, A.vterm, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5]);
+    }
+    A.out_cost[t0 + tid] = key_cost((uint32_t)(best >> 32));
+    A.out_pat[t0 + tid] = (int16_t)(uint32_t)best;
+    if (fb) atomicAdd(&nfb, 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    atomicAdd(&A.stats[0], (unsigned long long)nrow);
+    atomicAdd(&A.stats[1], (unsigned long long)nfb);
+  }
+}
+
+typedef void (*PropFn)(PropArgs);
+template <bool ROUTE, int TT>
+PropFn prop_kernel_t(int Z) {
+  switch (Z) {
+    case 3: return k_prop<3, ROUTE, TT>;
+    case 4: return k_prop<4, ROUTE, TT>;
+    case 5: return k_prop<5, ROUTE, TT>;
+    case 6: return k_prop<6, ROUTE, TT>;
+    case 7: return k_prop<7, ROUTE, TT>;
+    case 8: return k_prop<8, ROUTE, TT>;
+    case 9: return k_prop<9, ROUTE, TT>;
+    case 10: return k_prop<10, ROUTE, TT>;
+    case 11: return k_prop<11, ROUTE, TT>;  // 60 candidate patterns; Z = 12 has 66 (unfused path)
+    default: return nullptr;
+  }
+}
+template <bool ROUTE>
+PropFn prop_kernel(int Z, int tt) { return tt == 64 ? prop_kernel_t<ROUTE, 64>(Z) : prop_kernel_t<ROUTE, 128>(Z); }
+
 int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
 }  // namespace
@@ -357,6 +717,9 @@ struct gr_mlp {
   size_t oW1, ob1, oW2, ob2, oW3, ob3;
   float *d_master = nullptr, *d_mom = nullptr, *d_grad = nullptr, *d_loglut = nullptr, *d_loss = nullptr;
   uint16_t *W1b = nullptr, *W1t = nullptr, *W2b = nullptr, *W2t = nullptr, *W3b = nullptr, *W3t = nullptr;
+  // fused inference (hidden width kHid): weights in MFMA fragment order
+  uint16_t *W1p = nullptr, *W2p = nullptr, *W3p = nullptr;
+  int nf3 = 0;  // packed output fragments of 16 neurons (n_op / 16)
   int64_t cap_samples = 0, n_samples = 0;
   uint16_t* d_X = nullptr;
   int32_t* d_y = nullptr;
@@ -397,6 +760,29 @@ void refresh_bf16(gr_mlp* m, hipStream_t st) {
                      w + m->oW2, m->W2b, m->W2t);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->H * m->n_op, 256)), dim3(256), 0, st, m->H, m->n_op,
                      w + m->oW3, m->W3b, m->W3t);
+  if (m->W1p) {
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)m->k_in * m->H, 256)), dim3(256), 0, st, m->k_in, m->H,
+                       m->H / 16, w + m->oW1, m->W1p);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)m->H * m->H, 256)), dim3(256), 0, st, m->H, m->H, m->H / 16,
+                       w + m->oW2, m->W2p);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)m->H * m->nf3 * 16, 256)), dim3(256), 0, st, m->H, m->n_op,
+                       m->nf3, w + m->oW3, m->W3p);
+  }
+}
+
+// Arguments of the fused kernel common to routing and the forward pass.
+PropArgs prop_args(gr_mlp* m) {
+  gr_ctx* c = m->ctx;
+  PropArgs A{};
+  A.g = c->g;
+  A.W1p = m->W1p;
+  A.W2p = m->W2p;
+  A.W3p = m->W3p;
+  A.b1 = m->d_master + m->ob1;
+  A.b2 = m->d_master + m->ob2;
+  A.b3 = m->d_master + m->ob3;
+  A.n_op = m->n_op;
+  return A;
 }
 
 // Forward pass of rows x a_X -> a_H1, a_H2, a_L (fp32 logits).
@@ -441,6 +827,14 @@ void backward(gr_mlp* m, hipStream_t st, int rows, int n_real) {
 
 namespace grh {
 
+bool mlp_fused(const gr_mlp* m) { return m->W1p != nullptr; }
+
+void mlp_detach_all(gr_ctx* c) {
+  for (gr_mlp* m : c->models) m->ctx = nullptr;
+  c->models.clear();
+  c->prop = c->collect = nullptr;
+}
+
 // Exhaustive costing already ran; capture a strided sample of the batch's
 // pairs with their chosen patterns as labels.
 int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& bm) {
@@ -461,6 +855,28 @@ int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& bm) {
 // Learned costing of one batch: features -> MLP -> top-k exact costing.
 int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
   gr_mlp* m = c->prop;
+  if (m->W1p) {
+    PropArgs A = prop_args(m);
+    A.rem = c->d_rem;
+    A.loglut = m->d_loglut;
+    A.pfx = c->d_pfx;
+    A.vterm = c->d_vterm;
+    A.pins = c->d_pins;
+    A.pinofs = s.d_pinofs + bm.begin;
+    A.bpair = s.d_pair + bm.begin;
+    A.chunk_net = s.d_chunk + bm.chunk_ofs;
+    A.n = bm.n;
+    A.P = bm.P;
+    A.topk = c->prop_k;
+    A.thr = c->prop_thr;
+    A.out_cost = c->d_pair_cost;
+    A.out_pat = c->d_pair_pat;
+    A.ctr = c->d_ctr;
+    A.stats = (unsigned long long*)c->d_prop_stats;
+    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z, kTile), dim3(blocks_for(bm.P, kTile)), dim3(256), 0, c->stream, A);
+    HIPC(c, hipGetLastError());
+    return 0;
+  }
   int rc;
   if ((rc = mlp_reserve(m, bm.P))) return rc;
   hipLaunchKernelGGL(k_feat_batch, dim3(blocks_for(bm.P, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
@@ -532,8 +948,15 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
       hipMemset(m->d_mom, 0, o * 4) != hipSuccess ||
       hipMemcpy(m->d_loglut, loglut.data(), kLut * 4, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(c, GR_ENODEV, "model upload failed"));
+  if (m->H == kHid && prop_kernel<true>(c->g.Z, kTile)) {
+    m->nf3 = m->n_op / 16;  // every padded output column (the forward pass returns them all)
+    if ((rc = dalloc(c, &m->W1p, (size_t)m->k_in * m->H)) || (rc = dalloc(c, &m->W2p, (size_t)m->H * m->H)) ||
+        (rc = dalloc(c, &m->W3p, (size_t)m->H * m->n_op)))
+      return bail(rc);
+  }
   refresh_bf16(m, c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(c, GR_ENODEV, "model set-up failed"));
+  c->models.push_back(m);
   *out = m;
   return 0;
 }
@@ -542,13 +965,18 @@ void gr_mlp_destroy(gr_mlp* m) {
   if (!m) return;
   gr_ctx* c = m->ctx;
   if (c) {
+    (void)hipSetDevice(c->dev);
     (void)hipStreamSynchronize(c->stream);
-    if (c->prop == m) c->prop = nullptr;
+    if (c->prop == m) {
+      c->prop = nullptr;
+      drop_graph(c);
+    }
     if (c->collect == m) c->collect = nullptr;
+    c->models.erase(std::remove(c->models.begin(), c->models.end(), m), c->models.end());
   }
   void* ps[] = {m->d_master, m->d_mom, m->d_grad, m->d_loglut, m->d_loss, m->W1b, m->W1t, m->W2b, m->W2t, m->W3b,
                 m->W3t, m->d_X, m->d_y, m->a_X, m->a_H1, m->a_H2, m->a_L, m->a_dL, m->a_dZ2, m->a_dZ1, m->a_y,
-                m->d_perm};
+                m->d_perm, m->W1p, m->W2p, m->W3p};
   for (void* p : ps) (void)hipFree(p);
   delete m;
 }
@@ -561,6 +989,7 @@ int gr_mlp_shape(const gr_mlp* m, int32_t* s) {
 
 int gr_mlp_get_params(gr_mlp* m, float* out) {
   if (!m || !out) return GR_EINVAL;
+  if (!m->ctx) return GR_ESTATE;
   gr_ctx* c = m->ctx;
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(out, m->d_master, (size_t)m->n_params * 4, hipMemcpyDeviceToHost));
@@ -569,6 +998,7 @@ int gr_mlp_get_params(gr_mlp* m, float* out) {
 
 int gr_mlp_set_params(gr_mlp* m, const float* in) {
   if (!m || !in) return GR_EINVAL;
+  if (!m->ctx) return GR_ESTATE;
   gr_ctx* c = m->ctx;
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(m->d_master, in, (size_t)m->n_params * 4, hipMemcpyHostToDevice));
@@ -596,6 +1026,7 @@ int gr_mlp_clear_samples(gr_mlp* m) {
 
 int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss) {
   if (!m || epochs < 0 || batch < 32 || batch % 32) return GR_EINVAL;
+  if (!m->ctx) return GR_ESTATE;
   gr_ctx* c = m->ctx;
   (void)hipSetDevice(c->dev);
   int rc;
@@ -659,6 +1090,7 @@ int gr_mlp_features(gr_ctx* c, gr_mlp* m, const int32_t* src, const int32_t* dst
 
 int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
   if (!m || n < 0 || (n && (!x || !logits))) return GR_EINVAL;
+  if (!m->ctx) return GR_ESTATE;
   if (!n) return 0;
   gr_ctx* c = m->ctx;
   (void)hipSetDevice(c->dev);
@@ -668,7 +1100,16 @@ int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
   for (size_t i = 0; i < h.size(); ++i) h[i] = f2bf(x[i]);
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(m->a_X, h.data(), h.size() * 2, hipMemcpyHostToDevice));
-  forward(m, c->stream, n, m->a_X);
+  if (m->W1p) {  // the fused kernel routing uses, so its logits are the ones routing ranks
+    PropArgs A = prop_args(m);
+    A.X = m->a_X;
+    A.P = n;
+    A.logits = m->a_L;
+    hipLaunchKernelGGL(prop_kernel<false>(c->g.Z, kTile), dim3(blocks_for(n, kTile)), dim3(256), 0, c->stream, A);
+    HIPC(c, hipGetLastError());
+  } else {
+    forward(m, c->stream, n, m->a_X);
+  }
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(logits, m->a_L, (size_t)n * m->n_op * 4, hipMemcpyDeviceToHost));
   return 0;
@@ -676,6 +1117,7 @@ int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
 
 int gr_mlp_gradient(gr_mlp* m, const float* x, const int32_t* y, int32_t n, float* grad, float* loss) {
   if (!m || n <= 0 || !x || !y || !grad) return GR_EINVAL;
+  if (!m->ctx) return GR_ESTATE;
   gr_ctx* c = m->ctx;
   (void)hipSetDevice(c->dev);
   const int rows = (n + 31) / 32 * 32;
@@ -709,6 +1151,8 @@ int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
     if ((rc = dalloc(c, &c->d_prop_stats, 2))) return rc;
     HIPC(c, hipMemset(c->d_prop_stats, 0, 16));
   }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);  // the captured pass holds the old costing kernel and its arguments
   c->prop = m;
   c->prop_k = k;
   c->prop_thr = threshold;

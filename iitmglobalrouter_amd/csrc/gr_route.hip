@@ -42,6 +42,12 @@ void build_lut(int model, float* lut) {
 
 namespace grh {
 std::string g_err;
+
+void drop_graph(gr_ctx* c) {
+  Sched& s = c->sched;
+  if (s.exec) { (void)hipGraphExecDestroy(s.exec); s.exec = nullptr; }
+  s.graph_failed = false;
+}
 }  // namespace grh
 
 using namespace grh;
@@ -416,6 +422,7 @@ void gr_destroy(gr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  mlp_detach_all(c);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_mark); (void)hipFree(c->d_run_stamp);
   (void)hipFree(c->d_pfx); (void)hipFree(c->d_vterm); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
@@ -499,7 +506,7 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
   (void)hipSetDevice(c->dev);
   Sched& s = c->sched;
   const bool whole = first == 0 && count == (int32_t)s.meta.size() && count > 0;
-  if (whole && !c->timing && !c->prop && !c->collect && !s.graph_failed) {
+  if (whole && !c->timing && (!c->prop || mlp_fused(c->prop)) && !c->collect && !s.graph_failed) {
     if (!s.exec) {
       // capture the whole pass once: kernels only (no memset nodes), fixed
       // arguments; the batch stamp comes from the device counter

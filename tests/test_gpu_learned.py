@@ -169,3 +169,44 @@ def test_training_gains(gr, gpu):
     print(f"route cost exhaustive {exhaustive} untrained {untrained} ({gap_u:.4%}) trained {trained} ({gap_t:.4%})",
           flush=True)
     assert gap_t < gap_u
+
+
+def test_model_outlives_router(gr, gpu):
+    """A router destroyed first detaches its models: their calls then fail
+    cleanly and closing them frees only their own buffers."""
+    r, *_ = _router(gr)
+    m = gr.Model(r, seed=0)
+    r.close()
+    with pytest.raises(gr.RouterError):
+        m.params()
+    with pytest.raises(gr.RouterError):
+        m.forward(np.zeros((4, m.k_in), np.float32))
+    m.close()
+
+
+def test_proposal_graph_replay_matches_route_batch(gr, gpu):
+    """Whole-schedule passes with the proposal (captured once, then replayed)
+    route exactly as batch-by-batch calls; changing k re-captures."""
+    X, Y, Z = 96, 96, 10
+    r, cap, pfx, pins, ofst, ids = _router(gr, X=X, Y=Y, nets=2500)
+    m = gr.Model(r, seed=7)
+    m.set_params(_rand_params(m, np.random.default_rng(5), 4.0))
+    b_ = gr.Router(X, Y, Z, cap)
+    b_.set_nets(pfx, pins)
+    b_.set_schedule(ofst, ids)
+    mb = gr.Model(b_, seed=7)
+    mb.set_params(m.params())
+    for k in (4, 2):
+        r.set_proposal(m, k=k, threshold=0.3)
+        r.reset_demand()
+        outs = [r.route_batch(ids[ofst[b]:ofst[b + 1]]) for b in range(len(ofst) - 1)]
+        b_.set_proposal(mb, k=k, threshold=0.3)
+        for rep in range(3):
+            b_.reset_demand()
+            b_.route_schedule()
+            e, l, c = b_.schedule_outputs()
+            assert np.array_equal(e, np.concatenate([x[0] for x in outs])), (k, rep)
+            assert np.array_equal(l, np.concatenate([x[1] for x in outs])), (k, rep)
+            assert np.array_equal(b_.demand(), r.demand()), (k, rep)
+    r.set_proposal(None)
+    b_.set_proposal(None)
