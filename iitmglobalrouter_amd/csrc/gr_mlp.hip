@@ -18,6 +18,8 @@ using namespace grd;
 using namespace grh;
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -390,11 +392,19 @@ __global__ __launch_bounds__(256) void k_pack(int K, int N, int NF, const float*
   Wp[i] = n < N ? f2bf(W[(int64_t)k * N + n]) : (uint16_t)0;
 }
 
+// W^T fragments (f, 0) of the first K step, loaded ahead of mma_tile.
+template <int KS, int NF>
+__device__ __forceinline__ void mma_first(const uint16_t* __restrict__ wp, uint4 (&a0)[NF], int lane) {
+  const uint4* wv = reinterpret_cast<const uint4*>(wp) + lane;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) a0[f] = wv[f * KS * 64];
+}
+
 // acc[f][q] = sum over the K steps of W^T fragment (f, s) x X rows
-// prow0 + 16q .. + 16.
+// prow0 + 16q .. + 16; a0 holds the fragments of step 0 (mma_first).
 template <int KS, int NF, int PF>
-__device__ __forceinline__ void mma_tile(const uint16_t* __restrict__ wp, const uint16_t* act, int RS, int prow0,
-                                         v4f (&acc)[NF][PF], int lane) {
+__device__ __forceinline__ void mma_tile(const uint16_t* __restrict__ wp, const uint4 (&a0)[NF], const uint16_t* act,
+                                         int RS, int prow0, v4f (&acc)[NF][PF], int lane) {
   const uint4* wv = reinterpret_cast<const uint4*>(wp) + lane;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
@@ -405,7 +415,7 @@ __device__ __forceinline__ void mma_tile(const uint16_t* __restrict__ wp, const 
   for (int s = 0; s < KS; ++s) {
     uint4 a[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) a[f] = wv[(f * KS + s) * 64];
+    for (int f = 0; f < NF; ++f) a[f] = s == 0 ? a0[f] : wv[(f * KS + s) * 64];
 #pragma unroll
     for (int q = 0; q < PF; ++q) {
       const v8bf b = __builtin_bit_cast(v8bf, *(const uint4*)(bp + q * 16 * RS + s * 32));
@@ -431,9 +441,10 @@ __device__ __forceinline__ void store_hidden(const v4f (&acc)[NF][PF], const flo
       v1 = v1 > 0.f ? v1 : 0.f;
       v2 = v2 > 0.f ? v2 : 0.f;
       v3 = v3 > 0.f ? v3 : 0.f;
+      // two values per v_cvt_pk_bf16_f32, round to nearest even like f2bf
       uint2 pk;
-      pk.x = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
-      pk.y = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
+      pk.x = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){v0, v1}, v2bf));
+      pk.y = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){v2, v3}, v2bf));
       *reinterpret_cast<uint2*>(act + (16 * q + (lane & 15)) * RS + n) = pk;
     }
   }
@@ -485,6 +496,10 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
   const int64_t t0 = (int64_t)blockIdx.x * TT;
   const int nrow = (int)(A.P - t0 < TT ? A.P - t0 : TT);
   const Geo& g = A.g;
+  const uint16_t* w1 = A.W1p + (size_t)wv * 4 * (KIN / 32) * 512;
+  const uint16_t* w2 = A.W2p + (size_t)wv * 4 * (kHid / 32) * 512;
+  uint4 a0[4];  // first-step weight fragments of the next layer, in flight across the phase before it
+  mma_first<KIN / 32, 4>(w1, a0, lane);
 
   if (ROUTE) {
     if (blockIdx.x == 0 && tid == 0) *A.ctr += 1;  // this batch's stamp
@@ -559,11 +574,12 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
   // hidden layers
   {
     v4f acc[4][TT / 16];
-    mma_tile<KIN / 32, 4, TT / 16>(A.W1p + (size_t)wv * 4 * (KIN / 32) * 512, act, RS, 0, acc, lane);
+    mma_tile<KIN / 32, 4, TT / 16>(w1, a0, act, RS, 0, acc, lane);
+    mma_first<kHid / 32, 4>(w2, a0, lane);
     __syncthreads();
     store_hidden<4, TT / 16>(acc, A.b1, wv * 64, act, RS, lane);
     __syncthreads();
-    mma_tile<kHid / 32, 4, TT / 16>(A.W2p + (size_t)wv * 4 * (kHid / 32) * 512, act, RS, 0, acc, lane);
+    mma_tile<kHid / 32, 4, TT / 16>(w2, a0, act, RS, 0, acc, lane);
     __syncthreads();
     store_hidden<4, TT / 16>(acc, A.b2, wv * 64, act, RS, lane);
     __syncthreads();
@@ -573,7 +589,9 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
   // output neurons 16f + 4(l >> 4) + i (f < NFL, i < 4)
   constexpr int NFL = ROUTE ? NF3 : S::NFO;
   v4f acc[NFL][QW];
-  mma_tile<kHid / 32, NFL, QW>(A.W3p, act, RS, 16 * QW * wv, acc, lane);
+  uint4 a3[NFL];
+  mma_first<kHid / 32, NFL>(A.W3p, a3, lane);
+  mma_tile<kHid / 32, NFL, QW>(A.W3p, a3, act, RS, 16 * QW * wv, acc, lane);
   if (!ROUTE) {
 #pragma unroll
     for (int f = 0; f < NFL; ++f) {
