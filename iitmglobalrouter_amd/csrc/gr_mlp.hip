@@ -25,6 +25,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int kMaxTopK = 8;
+constexpr int kSplit = 8;  // row chunks of the weight-gradient products
 
 // bf16 <-> fp32 by bit manipulation (round to nearest even); identical on host and device.
 __host__ __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -222,14 +223,90 @@ __global__ __launch_bounds__(256) void k_softmax_ce(int rows, int n_out, int ld,
   atomicAdd(loss_sum, -(l[lab] - mx - logf(s)));
 }
 
-// db[j] = sum over rows of dZ[row][j] (bf16 values, fp32 sum in row order).
-__global__ __launch_bounds__(256) void k_colsum(int rows, int cols, const uint16_t* __restrict__ dZ, int64_t ld,
-                                                float* __restrict__ out) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= cols) return;
-  float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += bf2f(dZ[(int64_t)r * ld + j]);
-  out[j] = s;
+// Weight and bias gradients of one layer, split over the rows (split-K):
+// part z of C [(M + 1) x N] = sum over rows r in [z*rz, (z+1)*rz) of
+// A'[r][m] B[r][n], A' = [A | 1] (A: [R x M] row-major, ld lda; B: [R x N],
+// ld ldb; rz % 32 == 0), so row M of C is the bias gradient.  Tiles of 32
+// rows are loaded coalesced (8 consecutive columns per thread) and stored
+// transposed in LDS, where the MFMA fragments (8 consecutive rows of one
+// column) are 16-byte reads.  64 x 64 output tile per 256 threads, 2 x 2
+// wavefronts of 32 x 32, as k_gemm.
+__global__ __launch_bounds__(256) void k_gemm_tn(int M, int N, int rz, const uint16_t* __restrict__ A, int64_t lda,
+                                                 const uint16_t* __restrict__ B, int64_t ldb, float* __restrict__ Cp,
+                                                 int64_t ldc) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][r]
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][r]
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  const int64_t rbase = (int64_t)blockIdx.z * rz;
+  float* C = Cp + (int64_t)blockIdx.z * (M + 1) * ldc;
+  v4f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+  const int lr = t >> 3, lc = (t & 7) * 8;
+  const int gm = m0 + lc, gn = n0 + lc;
+  for (int r0 = 0; r0 < rz; r0 += 32) {
+    const int64_t row = rbase + r0 + lr;
+    uint16_t av[8], bv[8];
+    if (gm + 8 <= M) {
+      const uint4 v = *reinterpret_cast<const uint4*>(A + row * lda + gm);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) av[i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) av[i] = gm + i < M ? A[row * lda + gm + i] : (gm + i == M ? (uint16_t)0x3F80 : 0);
+    }
+    if (gn + 8 <= N) {
+      const uint4 v = *reinterpret_cast<const uint4*>(B + row * ldb + gn);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bv[i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bv[i] = gn + i < N ? B[row * ldb + gn + i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      As[lc + i][lr] = av[i];
+      Bs[lc + i][lr] = bv[i];
+    }
+    __syncthreads();
+    v8bf fa[2], fb[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      fa[f] = __builtin_bit_cast(v8bf, *(const uint4*)&As[wm + f * 16 + (lane & 15)][8 * (lane >> 4)]);
+      fb[f] = __builtin_bit_cast(v8bf, *(const uint4*)&Bs[wn + f * 16 + (lane & 15)][8 * (lane >> 4)]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + i;
+        const int col = n0 + wn + b * 16 + (lane & 15);
+        if (r <= M && col < N) C[(int64_t)r * ldc + col] = acc[a][b][i];
+      }
+}
+
+// out[i] = sum over z < nz of parts[z][i], in z order.
+__global__ __launch_bounds__(256) void k_sum_parts(int64_t n, int nz, const float* __restrict__ parts,
+                                                   float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s = parts[i];
+  for (int z = 1; z < nz; ++z) s += parts[(int64_t)z * n + i];
+  out[i] = s;
 }
 
 // Momentum SGD on the fp32 master parameters.
@@ -745,6 +822,7 @@ struct gr_mlp {
   uint16_t *a_X = nullptr, *a_H1 = nullptr, *a_H2 = nullptr, *a_dL = nullptr, *a_dZ2 = nullptr, *a_dZ1 = nullptr;
   float* a_L = nullptr;
   int32_t *a_y = nullptr, *d_perm = nullptr;
+  float* a_part = nullptr;  // split-K partial gradients [kSplit][largest layer's (M + 1) x N]
 };
 
 namespace {
@@ -756,7 +834,8 @@ int mlp_reserve(gr_mlp* m, int64_t rows) {
   if ((rc = dalloc(c, &m->a_X, rows * m->k_in)) || (rc = dalloc(c, &m->a_H1, rows * m->H)) ||
       (rc = dalloc(c, &m->a_H2, rows * m->H)) || (rc = dalloc(c, &m->a_L, rows * m->n_op)) ||
       (rc = dalloc(c, &m->a_dL, rows * m->n_op)) || (rc = dalloc(c, &m->a_dZ2, rows * m->H)) ||
-      (rc = dalloc(c, &m->a_dZ1, rows * m->H)) || (rc = dalloc(c, &m->a_y, rows)) || (rc = dalloc(c, &m->d_perm, rows)))
+      (rc = dalloc(c, &m->a_dZ1, rows * m->H)) || (rc = dalloc(c, &m->a_y, rows)) || (rc = dalloc(c, &m->d_perm, rows)) ||
+      (rc = dalloc(c, &m->a_part, (size_t)kSplit * (std::max(m->k_in, m->H) + 1) * std::max(m->H, m->n_op))))
     return rc;
   m->act_cap = rows;
   return 0;
@@ -814,31 +893,35 @@ void forward(gr_mlp* m, hipStream_t st, int64_t rows, const uint16_t* X) {
                     m->a_L, m->n_op);
 }
 
+// [dW; db] of one layer into g (the bias follows its weight in the parameter
+// layout, as row M of an (M + 1) x N matrix): split-K over the rows, then the
+// parts summed in order.
+void grad_tn(gr_mlp* m, hipStream_t st, int rows, int M, int N, const uint16_t* A, const uint16_t* B, float* g) {
+  int nz = kSplit;
+  while (nz > 1 && (rows % (32 * nz) || rows / nz < 64)) nz >>= 1;
+  hipLaunchKernelGGL(k_gemm_tn, dim3(blocks_for(M + 1, 64), blocks_for(N, 64), nz), dim3(256), 0, st, M, N, rows / nz,
+                     A, (int64_t)M, B, (int64_t)N, nz > 1 ? m->a_part : g, (int64_t)N);
+  if (nz > 1)
+    hipLaunchKernelGGL(k_sum_parts, dim3(blocks_for((int64_t)(M + 1) * N, 256)), dim3(256), 0, st,
+                       (int64_t)(M + 1) * N, nz, m->a_part, g);
+}
+
 // Loss and gradient of rows (multiple of 32) in a_X / a_y (padding labels -1);
 // gradient into d_grad, loss sum into d_loss.
 void backward(gr_mlp* m, hipStream_t st, int rows, int n_real) {
   float* gr = m->d_grad;
   hipLaunchKernelGGL(k_softmax_ce, dim3(blocks_for(rows, 256)), dim3(256), 0, st, rows, m->n_out, m->n_op, m->a_L,
                      m->a_y, 1.0f / (float)n_real, m->a_dL, m->d_loss);
-  // layer 3: dW3 = H2^T dL, db3 = colsum dL, dZ2 = (dL W3^T) * [H2 > 0]
-  gemm<false, true>(st, m->H, m->n_op, rows, m->a_H2, 1, m->H, m->a_dL, m->n_op, 1, nullptr, nullptr, 0, nullptr,
-                    gr + m->oW3, m->n_op);
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->n_op, 256)), dim3(256), 0, st, rows, m->n_op, m->a_dL,
-                     (int64_t)m->n_op, gr + m->ob3);
+  // layer 3: [dW3; db3] = [H2 | 1]^T dL, dZ2 = (dL W3^T) * [H2 > 0]
+  grad_tn(m, st, rows, m->H, m->n_op, m->a_H2, m->a_dL, gr + m->oW3);
   gemm<false, false>(st, rows, m->H, m->n_op, m->a_dL, m->n_op, 1, m->W3b, 1, m->n_op, nullptr, m->a_H2, m->H,
                      m->a_dZ2, nullptr, m->H);
   // layer 2
-  gemm<false, true>(st, m->H, m->H, rows, m->a_H1, 1, m->H, m->a_dZ2, m->H, 1, nullptr, nullptr, 0, nullptr,
-                    gr + m->oW2, m->H);
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->H, 256)), dim3(256), 0, st, rows, m->H, m->a_dZ2, (int64_t)m->H,
-                     gr + m->ob2);
+  grad_tn(m, st, rows, m->H, m->H, m->a_H1, m->a_dZ2, gr + m->oW2);
   gemm<false, false>(st, rows, m->H, m->H, m->a_dZ2, m->H, 1, m->W2b, 1, m->H, nullptr, m->a_H1, m->H, m->a_dZ1,
                      nullptr, m->H);
   // layer 1
-  gemm<false, true>(st, m->k_in, m->H, rows, m->a_X, 1, m->k_in, m->a_dZ1, m->H, 1, nullptr, nullptr, 0, nullptr,
-                    gr + m->oW1, m->H);
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(m->H, 256)), dim3(256), 0, st, rows, m->H, m->a_dZ1, (int64_t)m->H,
-                     gr + m->ob1);
+  grad_tn(m, st, rows, m->k_in, m->H, m->a_X, m->a_dZ1, gr + m->oW1);
 }
 
 }  // namespace
@@ -994,7 +1077,7 @@ void gr_mlp_destroy(gr_mlp* m) {
   }
   void* ps[] = {m->d_master, m->d_mom, m->d_grad, m->d_loglut, m->d_loss, m->W1b, m->W1t, m->W2b, m->W2t, m->W3b,
                 m->W3t, m->d_X, m->d_y, m->a_X, m->a_H1, m->a_H2, m->a_L, m->a_dL, m->a_dZ2, m->a_dZ1, m->a_y,
-                m->d_perm, m->W1p, m->W2p, m->W3p};
+                m->d_perm, m->W1p, m->W2p, m->W3p, m->a_part};
   for (void* p : ps) (void)hipFree(p);
   delete m;
 }
