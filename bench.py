@@ -226,6 +226,95 @@ def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
     return out
 
 
+def stress_eval(gr, args, tuning):
+    """Combined stress (2000x2000x10, capacities cut 40% plus 30 Gaussian
+    hotspots, 1.5M nets requested): the plain pass against the learned
+    proposal (k=4, 0.6 confidence threshold), first trained on exhaustive
+    routing of other seeds, then fine-tuned online every 50 batches on
+    fallback-judged decisions (pairs whose choice came from every pattern).
+    The learned pass is timed whole, fine-tuning included."""
+    W = gr.WORKLOADS[5]
+    X, Y, Z = W["X"], W["Y"], W["Z"]
+    t = time.perf_counter()
+    cap = gr.gen_grid(5, args.seed, X, Y, Z)
+    pfx, pins = gr.gen_nets(5, args.seed, X, Y, Z, W["nets"])
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    n_nets, nb = len(pfx) - 1, len(ofst) - 1
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    if tuning:
+        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave"]
+        r.set_tuning(**{k: tuning[k] for k in keys})
+    print(f"# stress setup {time.perf_counter() - t:.2f}s nets {n_nets} batches {nb}", file=sys.stderr, flush=True)
+    r.reset_demand()
+    r.route_schedule()  # warm-up; captures the pass
+    r.sync()
+    steps = 2
+    ta = time.perf_counter()
+    for _ in range(steps):
+        r.reset_demand()
+        r.route_schedule()
+    r.sync()
+    plain_ms = (time.perf_counter() - ta) * 1e3 / steps
+    plain_cost = gr.route_cost(cap, r.demand())
+    # initial model from exhaustive routing of seeds 1..stress_seeds
+    t = time.perf_counter()
+    m = gr.Model(r, hidden=256, seed=0, max_samples=1 << 20)
+    m.collect(args.train_per_batch)
+    for sd in range(1, args.stress_seeds + 1):
+        pf, pn = gr.gen_nets(5, sd, X, Y, Z, W["nets"])
+        of, ii = gr.partition_nets(X, Y, Z, pf, pn, 4)
+        r.set_nets(pf, pn)
+        r.set_schedule(of, ii)
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+    m.collect(0)
+    for e in range(args.train_epochs):
+        m.train(epochs=1, batch=1024, lr=0.05, seed=e)
+    m.clear_samples()
+    pretrain_s = time.perf_counter() - t
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    r.set_proposal(m, k=4, threshold=0.6)
+    m.collect(500)
+    r.reset_demand()
+    r.proposal_stats()
+    r.sync()
+    ta = time.perf_counter()
+    rounds = 0
+    for b0 in range(0, nb, 50):
+        r.route_schedule(b0, min(50, nb - b0))
+        if b0 + 50 < nb:  # fine-tune on this chunk's fallback-judged samples
+            try:
+                m.train(epochs=1, batch=1024, lr=0.02, seed=1000 + b0)
+                rounds += 1
+            except gr.RouterError:
+                pass  # fewer labelled samples than one minibatch
+            m.clear_samples()
+    r.sync()
+    learned_ms = (time.perf_counter() - ta) * 1e3
+    m.collect(0)
+    pairs, fb = r.proposal_stats()
+    learned_cost = gr.route_cost(cap, r.demand())
+    r.set_proposal(None)
+    m.close()
+    r.close()
+    return {
+        "workload": f"{X}x{Y}x{Z} GCELLs, capacities cut 40% plus 30 hotspots, {W['nets']} nets requested "
+                    f"(seed {args.seed}), {n_nets} routed, {nb} batches",
+        "plain": {"nets_per_s": round(n_nets / (plain_ms / 1e3), 1), "ms_per_pass": round(plain_ms, 3),
+                  "route_cost": plain_cost},
+        "learned_online": {
+            "nets_per_s": round(n_nets / (learned_ms / 1e3), 1), "ms_per_pass": round(learned_ms, 3),
+            "vs_plain": round(plain_ms / learned_ms, 4), "route_cost": learned_cost,
+            "route_cost_gap": round((learned_cost["total"] - plain_cost["total"]) / plain_cost["total"], 6),
+            "fallback_rate": round(fb / max(pairs, 1), 4), "finetune_rounds": rounds,
+            "pretrain": {"seeds": args.stress_seeds, "s": round(pretrain_s, 2)}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +330,9 @@ def main():
     ap.add_argument("--tuning", default="", help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave")
     ap.add_argument("--io", type=int, default=1, help="also measure evaluator, writer and reader (0: skip)")
     ap.add_argument("--mst", type=int, default=1, help="also time true-MST trees for route quality (0: skip)")
+    ap.add_argument("--stress", type=int, default=1,
+                    help="also run the combined-stress workload (learned proposal with online fine-tuning; 0: skip)")
+    ap.add_argument("--stress-seeds", type=int, default=2, help="seeds of exhaustive routing for the stress model")
     ap.add_argument("--train-seeds", type=int, default=20)
     ap.add_argument("--train-per-batch", type=int, default=1500)
     ap.add_argument("--train-epochs", type=int, default=4)
@@ -344,6 +436,10 @@ def main():
     learned = None
     if args.learned:
         learned = learned_eval(gr, r, X, Y, Z, cap, W["nets"], args, n_nets, ms_step, (pfx, pins, ofst, ids))
+    stress = None
+    if args.stress:
+        stress = stress_eval(gr, args, tuning)
+        print(f"# stress {stress}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, args.cpu_seconds)
@@ -360,6 +456,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kstats,
         "learned": learned,
+        "stress": stress,
         "true_mst": true_mst,
         "callers": io,
         "path_bytes_per_step": bytes_path,

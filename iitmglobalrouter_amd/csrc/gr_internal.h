@@ -89,6 +89,7 @@ struct gr_ctx {
   grh::Sched sched, single;
   float* d_pair_cost = nullptr;
   int16_t* d_pair_pat = nullptr;
+  uint8_t* d_pair_fb = nullptr;  // per pair of the batch: 1 when the proposal fell back to every pattern
   int64_t pair_cap = 0, last_P = 0;
   // learned proposal: when set, costing evaluates only the model's top-k patterns
   gr_mlp* prop = nullptr;

@@ -86,14 +86,16 @@ __device__ void feat_row(const Geo& g, const int32_t* __restrict__ rem, const fl
 
 // Features of every pair of a batch (costing) or of a strided sample of them
 // (training capture: sample i is pair i*stride; its label is that pair's
-// exhaustive pattern minus Z2).
+// chosen pattern minus Z2, or -1 when fb is given and the pair's choice came
+// from the proposal's candidates rather than from every pattern).
 __global__ __launch_bounds__(256) void k_feat_batch(Geo g, const int32_t* __restrict__ rem,
                                                     const float* __restrict__ loglut,
                                                     const int32_t* __restrict__ pins,
                                                     const int64_t* __restrict__ pinofs,
                                                     const int64_t* __restrict__ bpair, int n, int64_t count,
                                                     int64_t stride, uint16_t* __restrict__ X, int k_in,
-                                                    const int16_t* __restrict__ pat, int32_t* __restrict__ y) {
+                                                    const int16_t* __restrict__ pat, int32_t* __restrict__ y,
+                                                    const uint8_t* __restrict__ fb) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= count) return;
   const int64_t w = i * stride;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(256) void k_feat_batch(Geo g, const int32_t* __rest
   pair_decode((uint32_t)(w - bpair[lo]), j, k);
   const int64_t pb = pinofs[lo];
   feat_row(g, rem, loglut, (uint32_t)pins[pb + j], (uint32_t)pins[pb + k], X + i * k_in, k_in);
-  if (y) y[i] = (int32_t)pat[w] - g.Z2;
+  if (y) y[i] = (fb && !fb[w]) ? -1 : (int32_t)pat[w] - g.Z2;
 }
 
 __global__ __launch_bounds__(256) void k_feat_pairs(Geo g, const int32_t* __restrict__ rem,
@@ -550,6 +552,7 @@ struct PropArgs {
   int16_t* out_pat;
   int32_t* ctr;
   unsigned long long* stats;
+  uint8_t* fbout;  // per pair: fell back to every pattern (sample capture), or null
 };
 
 // ROUTE: features of the batch's pairs gathered from the grid, top-k selection
@@ -773,6 +776,7 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
     }
     A.out_cost[t0 + tid] = key_cost((uint32_t)(best >> 32));
     A.out_pat[t0 + tid] = (int16_t)(uint32_t)best;
+    if (A.fbout) A.fbout[t0 + tid] = fb ? 1 : 0;
     if (fb) atomicAdd(&nfb, 1);
   }
   __syncthreads();
@@ -947,7 +951,8 @@ int mlp_enqueue_collect(gr_ctx* c, Sched& s, const BatchMeta& bm) {
   want = std::min<int64_t>(want, (bm.P + stride - 1) / stride);
   hipLaunchKernelGGL(k_feat_batch, dim3(blocks_for(want, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
                      c->d_pins, s.d_pinofs + bm.begin, s.d_pair + bm.begin, bm.n, want, stride,
-                     m->d_X + m->n_samples * m->k_in, m->k_in, (const int16_t*)c->d_pair_pat, m->d_y + m->n_samples);
+                     m->d_X + m->n_samples * m->k_in, m->k_in, (const int16_t*)c->d_pair_pat, m->d_y + m->n_samples,
+                     c->prop ? (const uint8_t*)c->d_pair_fb : nullptr);
   m->n_samples += want;
   HIPC(c, hipGetLastError());
   return 0;
@@ -974,6 +979,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.out_pat = c->d_pair_pat;
     A.ctr = c->d_ctr;
     A.stats = (unsigned long long*)c->d_prop_stats;
+    A.fbout = c->collect ? c->d_pair_fb : nullptr;
     hipLaunchKernelGGL(prop_kernel<true>(c->g.Z, kTile), dim3(blocks_for(bm.P, kTile)), dim3(256), 0, c->stream, A);
     HIPC(c, hipGetLastError());
     return 0;
@@ -982,7 +988,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
   if ((rc = mlp_reserve(m, bm.P))) return rc;
   hipLaunchKernelGGL(k_feat_batch, dim3(blocks_for(bm.P, 256)), dim3(256), 0, c->stream, c->g, c->d_rem, m->d_loglut,
                      c->d_pins, s.d_pinofs + bm.begin, s.d_pair + bm.begin, bm.n, bm.P, (int64_t)1, m->a_X, m->k_in,
-                     (const int16_t*)nullptr, (int32_t*)nullptr);
+                     (const int16_t*)nullptr, (int32_t*)nullptr, (const uint8_t*)nullptr);
   forward(m, c->stream, bm.P, m->a_X);
   hipLaunchKernelGGL(k_lpat_topk, dim3(blocks_for(bm.P, 256)), dim3(256), 0, c->stream, c->g, c->d_pfx, c->d_pins,
                      s.d_pinofs + bm.begin, s.d_pair + bm.begin, s.d_chunk + bm.chunk_ofs, bm.n, bm.P, m->a_L, m->n_op,
@@ -1111,7 +1117,8 @@ int gr_mlp_set_params(gr_mlp* m, const float* in) {
 int gr_mlp_collect(gr_ctx* c, gr_mlp* m, int32_t per_batch) {
   if (!c || (per_batch > 0 && (!m || m->ctx != c)) || per_batch < 0)
     return fail(c, GR_EINVAL, "bad collect request (the model must belong to this context)");
-  if (per_batch > 0 && c->prop) return fail(c, GR_ESTATE, "collect labels from exhaustive costing: clear the proposal first");
+  if (per_batch > 0 && c->prop && !mlp_fused(c->prop))
+    return fail(c, GR_ESTATE, "capture under a proposal needs the fused kernel's fallback flags (hidden width 256)");
   c->collect = per_batch > 0 ? m : nullptr;
   c->collect_per_batch = per_batch;
   return 0;
@@ -1132,14 +1139,21 @@ int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t se
   (void)hipSetDevice(c->dev);
   int rc;
   if ((rc = mlp_reserve(m, batch))) return rc;
-  const int64_t S = m->n_samples;
-  if (S < batch) return fail(c, GR_ESTATE, "fewer samples than one minibatch");
+  // samples labelled -1 (capture under a proposal, choice not from every
+  // pattern) are left out
+  std::vector<int32_t> lab(m->n_samples), keep;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (m->n_samples) HIPC(c, hipMemcpy(lab.data(), m->d_y, 4 * lab.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < m->n_samples; ++i)
+    if (lab[i] >= 0) keep.push_back((int32_t)i);
+  const int64_t S = (int64_t)keep.size();
+  if (S < batch) return fail(c, GR_ESTATE, "fewer labelled samples than one minibatch");
   std::mt19937_64 rng(seed);
   std::vector<int32_t> perm(S);
   float last = 0.f;
   const int steps = (int)(S / batch);
   for (int e = 0; e < epochs; ++e) {
-    for (int64_t i = 0; i < S; ++i) perm[i] = (int32_t)i;
+    perm = keep;
     std::shuffle(perm.begin(), perm.end(), rng);
     HIPC(c, hipMemsetAsync(m->d_loss, 0, 4, c->stream));
     for (int st = 0; st < steps; ++st) {
@@ -1245,7 +1259,8 @@ int gr_mlp_gradient(gr_mlp* m, const float* x, const int32_t* y, int32_t n, floa
 int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
   if (!c) return fail(c, GR_EINVAL, "null context");
   if (m && (k < 1 || k > kMaxTopK || m->ctx != c)) return fail(c, GR_EINVAL, "k must be 1..8 and the model this context's");
-  if (m && c->collect) return fail(c, GR_ESTATE, "stop sample collection before routing with the proposal");
+  if (m && c->collect && !mlp_fused(m))
+    return fail(c, GR_ESTATE, "capture under a proposal needs the fused kernel's fallback flags (hidden width 256)");
   (void)hipSetDevice(c->dev);
   if (!c->d_prop_stats) {
     int rc;

@@ -176,7 +176,9 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     s.cap_chunk = chunk.size();
   }
   if (maxP > c->pair_cap) {
-    if ((rc = dalloc(c, &c->d_pair_cost, maxP)) || (rc = dalloc(c, &c->d_pair_pat, maxP))) return rc;
+    if ((rc = dalloc(c, &c->d_pair_cost, maxP)) || (rc = dalloc(c, &c->d_pair_pat, maxP)) ||
+        (rc = dalloc(c, &c->d_pair_fb, maxP)))
+      return rc;
     c->pair_cap = maxP;
   }
   if (npos) {
@@ -429,7 +431,7 @@ void gr_destroy(gr_ctx* c) {
   (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
   (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
-  (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat);
+  (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat); (void)hipFree(c->d_pair_fb);
   free_sched(c->sched);
   free_sched(c->single);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

@@ -231,7 +231,9 @@ int gr_kernel_stats(gr_ctx* ctx, int64_t* launches, double* total_ms);
  * (Z-1)*Z2 candidates (patterns with first layer a >= 1).  Weights are bf16 on
  * MFMA with fp32 accumulation and an fp32 master copy; training runs on the
  * GPU with the library's own forward and backward kernels on samples captured
- * from exhaustive routing (the chosen pattern is the label).
+ * from exhaustive routing (the chosen pattern is the label), or online from
+ * routing with the proposal, where only the pairs that fell back to every
+ * pattern carry a label.
  */
 typedef struct gr_mlp gr_mlp;
 
@@ -244,11 +246,14 @@ int gr_mlp_shape(const gr_mlp* m, int32_t* shape6);
 int gr_mlp_get_params(gr_mlp* m, float* out);
 int gr_mlp_set_params(gr_mlp* m, const float* in);
 /* While per_batch > 0, every routed batch appends up to per_batch samples
- * (features at the state before its commit, label = exhaustive choice). */
+ * (features at the state before its commit, label = the chosen pattern).
+ * Under a proposal (hidden width 256) a sample's label is kept only when its
+ * pair fell back to every pattern (the exhaustive choice); the others are
+ * stored with label -1 and training skips them. */
 int gr_mlp_collect(gr_ctx* ctx, gr_mlp* m, int32_t per_batch);
 int64_t gr_mlp_samples(const gr_mlp* m);
 int gr_mlp_clear_samples(gr_mlp* m);
-/* SGD with momentum 0.9 over the samples; *loss = mean cross-entropy of the last epoch. */
+/* SGD with momentum 0.9 over the labelled samples; *loss = mean cross-entropy of the last epoch. */
 int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss);
 /* Features of explicit pairs (src/dst cell encodings) at the current state: n x k_in. */
 int gr_mlp_features(gr_ctx* ctx, gr_mlp* m, const int32_t* src, const int32_t* dst, int32_t n, float* out);

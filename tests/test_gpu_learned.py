@@ -210,3 +210,45 @@ def test_proposal_graph_replay_matches_route_batch(gr, gpu):
             assert np.array_equal(b_.demand(), r.demand()), (k, rep)
     r.set_proposal(None)
     b_.set_proposal(None)
+
+
+def test_capture_under_proposal_keeps_fallback_labels(gr, gpu):
+    """Capture while routing with the proposal keeps labels only where the pair
+    fell back to every pattern: with every pair falling back (threshold 2) it
+    equals capture from exhaustive routing, so training on either gives the
+    same parameters bit for bit; with no low-confidence fallback almost no
+    sample is labelled."""
+    X, Y, Z = 96, 96, 10
+    a, cap, pfx, pins, ofst, ids = _router(gr, X=X, Y=Y, nets=2500)
+    b = gr.Router(X, Y, Z, cap)
+    b.set_nets(pfx, pins)
+    ma = gr.Model(a, seed=3, max_samples=1 << 15)
+    mb = gr.Model(b, seed=3, max_samples=1 << 15)
+    ma.collect(200)
+    mb.collect(200)
+    b.set_proposal(mb, k=4, threshold=2.0)
+    for i in range(len(ofst) - 1):
+        a.route_batch(ids[ofst[i]:ofst[i + 1]])
+        b.route_batch(ids[ofst[i]:ofst[i + 1]])
+    ma.collect(0)
+    mb.collect(0)
+    assert ma.samples() == mb.samples() > 256
+    la = ma.train(epochs=1, batch=64, lr=0.05, seed=1)
+    lb = mb.train(epochs=1, batch=64, lr=0.05, seed=1)
+    assert la == lb
+    assert np.array_equal(ma.params(), mb.params())
+    # confident proposal: labels only on the rare pairs without a valid candidate
+    mb.clear_samples()
+    mb.collect(200)
+    b.set_proposal(mb, k=4, threshold=0.0)
+    b.proposal_stats()  # counters from here on
+    b.reset_demand()
+    for i in range(len(ofst) - 1):
+        b.route_batch(ids[ofst[i]:ofst[i + 1]])
+    mb.collect(0)
+    pairs, fb = b.proposal_stats()
+    assert mb.samples() > 256
+    if fb < 32:
+        with pytest.raises(gr.RouterError):
+            mb.train(epochs=1, batch=32, lr=0.05, seed=1)
+    b.set_proposal(None)
