@@ -199,30 +199,30 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
 
 // Softmax cross-entropy over the first n_out logits of each row; labels < 0
 // mark padding rows.  dL = (softmax - onehot) / n_real as bf16 (padding
-// columns 0).  loss_sum += -log p[y] (sum over rows).
+// columns 0).  loss_sum += -log p[y] (float atomics over rows: the sum's last
+// bits depend on their order; the gradient does not).  One wavefront per row,
+// lane j holding columns j and j + 64 (n_out <= 128).
 __global__ __launch_bounds__(256) void k_softmax_ce(int rows, int n_out, int ld, const float* __restrict__ L,
                                                     const int32_t* __restrict__ y, float inv_n,
                                                     uint16_t* __restrict__ dL, float* __restrict__ loss_sum) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= rows) return;
   const float* l = L + (int64_t)r * ld;
   uint16_t* d = dL + (int64_t)r * ld;
   const int lab = y[r];
-  if (lab < 0) {
-    for (int j = 0; j < ld; ++j) d[j] = 0;
-    return;
-  }
-  float mx = l[0];
-  for (int j = 1; j < n_out; ++j) mx = fmaxf(mx, l[j]);
-  float s = 0.f;
-  for (int j = 0; j < n_out; ++j) s += expf(l[j] - mx);
-  const float inv = 1.f / s;
-  for (int j = 0; j < n_out; ++j) {
-    const float p = expf(l[j] - mx) * inv;
-    d[j] = f2bf((p - (j == lab ? 1.f : 0.f)) * inv_n);
-  }
-  for (int j = n_out; j < ld; ++j) d[j] = 0;
-  atomicAdd(loss_sum, -(l[lab] - mx - logf(s)));
+  const bool v0 = lane < n_out, v1 = lane + 64 < n_out;
+  const float x0 = v0 ? l[lane] : -INFINITY, x1 = v1 ? l[lane + 64] : -INFINITY;
+  float mx = fmaxf(x0, x1);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  const float e0 = v0 ? expf(x0 - mx) : 0.f, e1 = v1 ? expf(x1 - mx) : 0.f;
+  float sm = e0 + e1;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) sm += __shfl_xor(sm, o, 64);
+  const float inv = 1.f / sm;
+  if (lane < ld) d[lane] = lab < 0 || !v0 ? (uint16_t)0 : f2bf((e0 * inv - (lane == lab ? 1.f : 0.f)) * inv_n);
+  if (lane + 64 < ld) d[lane + 64] = lab < 0 || !v1 ? (uint16_t)0 : f2bf((e1 * inv - (lane + 64 == lab ? 1.f : 0.f)) * inv_n);
+  if (lane == 0 && lab >= 0) atomicAdd(loss_sum, -(l[lab] - mx - logf(sm)));
 }
 
 // Weight and bias gradients of one layer, split over the rows (split-K):
@@ -853,7 +853,9 @@ void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, i
                      A, sam, sak, B, sbk, sbn, bias, mask, ldm, Cb, Cf, ldc);
 }
 
-void refresh_bf16(gr_mlp* m, hipStream_t st) {
+// bf16 copies of the master weights for the training GEMMs; with pack, also
+// the fragment-order copies the fused kernel reads (routing, forward).
+void refresh_bf16(gr_mlp* m, hipStream_t st, bool pack = true) {
   const float* w = m->d_master;
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->k_in * m->H, 256)), dim3(256), 0, st, m->k_in, m->H,
                      w + m->oW1, m->W1b, m->W1t);
@@ -861,7 +863,7 @@ void refresh_bf16(gr_mlp* m, hipStream_t st) {
                      w + m->oW2, m->W2b, m->W2t);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((int64_t)m->H * m->n_op, 256)), dim3(256), 0, st, m->H, m->n_op,
                      w + m->oW3, m->W3b, m->W3t);
-  if (m->W1p) {
+  if (m->W1p && pack) {
     hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)m->k_in * m->H, 256)), dim3(256), 0, st, m->k_in, m->H,
                        m->H / 16, w + m->oW1, m->W1p);
     hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)m->H * m->H, 256)), dim3(256), 0, st, m->H, m->H, m->H / 16,
@@ -914,7 +916,7 @@ void grad_tn(gr_mlp* m, hipStream_t st, int rows, int M, int N, const uint16_t* 
 // gradient into d_grad, loss sum into d_loss.
 void backward(gr_mlp* m, hipStream_t st, int rows, int n_real) {
   float* gr = m->d_grad;
-  hipLaunchKernelGGL(k_softmax_ce, dim3(blocks_for(rows, 256)), dim3(256), 0, st, rows, m->n_out, m->n_op, m->a_L,
+  hipLaunchKernelGGL(k_softmax_ce, dim3(blocks_for(rows, 4)), dim3(256), 0, st, rows, m->n_out, m->n_op, m->a_L,
                      m->a_y, 1.0f / (float)n_real, m->a_dL, m->d_loss);
   // layer 3: [dW3; db3] = [H2 | 1]^T dL, dZ2 = (dL W3^T) * [H2 > 0]
   grad_tn(m, st, rows, m->H, m->n_op, m->a_H2, m->a_dL, gr + m->oW3);
@@ -1006,6 +1008,7 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
   if (!c || !out || hidden < 32 || hidden % 32 || max_samples < 0) return fail(c, GR_EINVAL, "bad model shape");
   *out = nullptr;
   if (c->d_remf) return fail(c, GR_ESTATE, "the learned proposal needs a grid with integer capacities");
+  if ((c->g.Z - 1) * c->g.Z2 > 128) return fail(c, GR_EINVAL, "the learned proposal scores at most 128 patterns (Z <= 16)");
   (void)hipSetDevice(c->dev);
   gr_mlp* m = new gr_mlp();
   m->ctx = c;
@@ -1165,13 +1168,15 @@ int gr_mlp_train(gr_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t se
       backward(m, c->stream, batch, batch);
       hipLaunchKernelGGL(k_sgd, dim3(blocks_for(m->n_params, 256)), dim3(256), 0, c->stream, m->n_params,
                          m->d_master, m->d_mom, m->d_grad, lr);
-      refresh_bf16(m, c->stream);
+      refresh_bf16(m, c->stream, false);
       HIPC(c, hipStreamSynchronize(c->stream));  // perm chunk is a host buffer
     }
     float ls = 0.f;
     HIPC(c, hipMemcpy(&ls, m->d_loss, 4, hipMemcpyDeviceToHost));
     last = ls / (float)((int64_t)steps * batch);
   }
+  refresh_bf16(m, c->stream);  // fragment-order copies for routing
+  HIPC(c, hipStreamSynchronize(c->stream));
   if (loss) *loss = last;
   return 0;
 }

@@ -235,7 +235,7 @@ def test_capture_under_proposal_keeps_fallback_labels(gr, gpu):
     assert ma.samples() == mb.samples() > 256
     la = ma.train(epochs=1, batch=64, lr=0.05, seed=1)
     lb = mb.train(epochs=1, batch=64, lr=0.05, seed=1)
-    assert la == lb
+    assert abs(la - lb) <= 1e-6 * abs(la)  # the reported loss is summed with atomics
     assert np.array_equal(ma.params(), mb.params())
     # confident proposal: labels only on the rare pairs without a valid candidate
     mb.clear_samples()

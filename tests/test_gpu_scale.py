@@ -1,7 +1,9 @@
 """GPU parity at scale: nets of hundreds of pins (pair costs streamed from
 global memory in the tree stage) and the full 2000x2000x10 / 1.5M-net
-workload, where sampled batches are re-routed by the oracle from the GPU's own
-state and must match bit-exactly."""
+workloads (heavy-tailed pin counts; and the same nets on capacities cut by 40%
+with 30 hotspots of capacity 1-2, where overflow is heavy), where sampled
+batches are re-routed by the oracle from the GPU's own state and must match
+bit-exactly."""
 import numpy as np
 import pytest
 
@@ -44,10 +46,11 @@ def test_huge_nets(gr, orc, gpu):
     assert np.array_equal(r.demand(), o.demand())
 
 
-def test_workload3_sampled_batches(gr, orc, gpu):
+@pytest.mark.parametrize("kind", [3, 5])
+def test_2000_sampled_batches(gr, orc, gpu, kind):
     X, Y, Z = 2000, 2000, 10
-    cap = gr.gen_grid(3, 0, X, Y, Z)
-    pfx, pins = gr.gen_nets(3, 0, X, Y, Z, 1500000)
+    cap = gr.gen_grid(kind, 0, X, Y, Z)
+    pfx, pins = gr.gen_nets(kind, 0, X, Y, Z, 1500000)
     ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
     nb = len(ofst) - 1
     deg = np.diff(pfx)
@@ -59,7 +62,7 @@ def test_workload3_sampled_batches(gr, orc, gpu):
     done = 0
     slot0 = np.concatenate([[0], np.cumsum(deg[ids])])
     for b in (1, nb // 2, nb - 1):
-        print(f"workload3 batch {b} of {nb}: {ofst[b + 1] - ofst[b]} nets", flush=True)
+        print(f"workload{kind} batch {b} of {nb}: {ofst[b + 1] - ofst[b]} nets", flush=True)
         r.route_schedule(done, b - done)
         r.sync()
         o.set_rem(cap - r.demand())
@@ -82,3 +85,5 @@ def test_workload3_sampled_batches(gr, orc, gpu):
     last = slot0[1:] - 1
     assert np.all(e[last] == -1) and np.all(l[last] == -1)
     assert int((e >= 0).sum()) == int((deg - 1).sum())
+    if kind == 5:  # the stress grid really overflows
+        assert int(np.maximum(r.demand() - cap, 0).sum()) > 0
