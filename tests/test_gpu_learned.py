@@ -71,9 +71,12 @@ def test_features_match_cpu(gr, orc, gpu):
     assert np.all(np.abs(got - want) <= np.abs(want) * 2.0 ** -7 + 1e-6)
 
 
-def test_always_fallback_is_exhaustive(gr, gpu):
-    a, cap, pfx, pins, ofst, ids = _router(gr, X=96, Y=96, nets=2500)
-    b = gr.Router(96, 96, 10, cap)
+# Z = 3..11 route through the fused kernel (one instantiation per Z); Z = 12
+# scores 66 patterns and takes the feature / GEMM / top-k kernel sequence
+@pytest.mark.parametrize("Z", [4, 10, 11, 12])
+def test_always_fallback_is_exhaustive(gr, gpu, Z):
+    a, cap, pfx, pins, ofst, ids = _router(gr, X=96, Y=96, Z=Z, nets=2500)
+    b = gr.Router(96, 96, Z, cap)
     b.set_nets(pfx, pins)
     m = gr.Model(b, seed=5)
     b.set_proposal(m, k=4, threshold=2.0)  # top-1 probability never reaches 2
@@ -87,16 +90,17 @@ def test_always_fallback_is_exhaustive(gr, gpu):
     assert pairs > 0 and fb == pairs
 
 
-def test_topk_choice_matches_oracle(gr, orc, gpu):
+@pytest.mark.parametrize("Z", [4, 10, 11, 12])
+def test_topk_choice_matches_oracle(gr, orc, gpu, Z):
     """With no fallback, each pair's pattern is the (cost, p) minimum over the
     model's top-k candidates, costed by the oracle at the same state."""
-    X, Y, Z = 96, 96, 10
-    r, cap, pfx, pins, ofst, ids = _router(gr, X=X, Y=Y, nets=2500)
+    X, Y = 96, 96
+    r, cap, pfx, pins, ofst, ids = _router(gr, X=X, Y=Y, Z=Z, nets=2500)
     m = gr.Model(r, seed=7)
     rng = np.random.default_rng(3)
     m.set_params(_rand_params(m, rng, 4.0))
     o = orc.Oracle(X, Y, Z, cap)
-    k = 3
+    k = {4: 2, 10: 3, 11: 8, 12: 5}[Z]
     r.set_proposal(m, k=k, threshold=0.0)
     Z2 = (Z + 1) // 2
     for b in range(2):
