@@ -184,7 +184,8 @@ def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
     r.sync()
     exhaustive = gr.route_cost(cap, r.demand())
     P = float((np.diff(pfx) * (np.diff(pfx) - 1) // 2).sum())
-    flop_pair = 2.0 * (m.k_in * m.hidden + m.hidden * m.hidden + m.hidden * m.n_out_pad)
+    # algorithmic MLP FLOPs per scored pair: 2 (254*256 + 256*256 + 256*45) = 284,160 at Z = 10
+    flop_pair = 2.0 * (m.n_in * m.hidden + m.hidden * m.hidden + m.hidden * m.n_out)
     out = {"train": {"seeds": args.train_seeds, "samples": int(m.samples()), "epochs": args.train_epochs,
                      "loss_per_epoch": [round(v, 4) for v in losses], "collect_s": round(t1 - t0, 2),
                      "train_s": round(t2 - t1, 2)},
@@ -221,7 +222,12 @@ def learned_eval(gr, r, X, Y, Z, cap, n_req, args, n_nets, plain_ms, seed0):
             "vs_plain": round(plain_ms / ms_l, 4), "route_cost": cost,
             "route_cost_gap": round((cost["total"] - exhaustive["total"]) / exhaustive["total"], 6),
             "fallback_rate": round(fb / max(pairs, 1), 4), "costing_ms_per_step": round(cost_ms, 3),
-            "mlp_TFLOPs_lower_bound": round(P * flop_pair / (cost_ms / 1e3) / 1e12, 2)}
+            "mlp_TFLOPs_lower_bound": round(P * flop_pair / (cost_ms / 1e3) / 1e12, 2),
+            # the fused costing kernel against the dense bf16 MFMA peak: its MLP FLOPs over its whole time
+            # (gather, top-k and candidate costing included), HIP events on the launch stream
+            "roofline": {"kernel": "k_prop", "bound": "mfma",
+                         "achieved": round(P * flop_pair / (cost_ms / 1e3) / 1e12, 2), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(P * flop_pair / (cost_ms / 1e3) / 1e12 / 2500.0, 4)}}
     m.close()
     return out
 
