@@ -71,6 +71,14 @@ def test_odd_layers_and_rectangular(gr, orc, gpu):
     _compare_batches(r, o, ofst, ids, every_demand=False, label="z7")
 
 
+@pytest.mark.parametrize("Z", [3, 4, 5, 8, 9, 11, 12, 13, 16])
+def test_layer_counts(gr, orc, gpu, Z):
+    """Every lane-kernel instantiation (Z = 3..12) and the wavefront-per-pair
+    kernel (Z > 12), every batch bit-exact on a small non-square grid."""
+    r, o, ofst, ids = _setup(gr, orc, 2, 20 + Z, 72, 56, Z, 900)
+    _compare_batches(r, o, ofst, ids, every_demand=False, label=f"z{Z}")
+
+
 def test_workload2_slice(gr, orc, gpu):
     """800x800x10 grid with 5% blocked cells and the workload-2 pin mix (incl. 21-200 pins)."""
     r, o, ofst, ids = _setup(gr, orc, 2, 0, 800, 800, 10, 6000)
