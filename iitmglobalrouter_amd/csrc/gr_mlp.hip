@@ -1,9 +1,12 @@
 // Learned proposal for L-pattern costing (no counterpart in the reference):
 // a 3-layer MLP scores the candidate patterns of each pin pair so that only
-// its top-k are costed exactly.  Hand-written gfx950 kernels: per-pair feature
-// gather, a bf16 MFMA GEMM (v_mfma_f32_16x16x32_bf16, fp32 accumulation) used
-// for the forward pass and every backward product, softmax cross-entropy,
-// momentum SGD on fp32 master weights, and the top-k costing kernel.
+// its top-k are costed exactly.  Hand-written gfx950 kernels, all on bf16 MFMA
+// (v_mfma_f32_16x16x32_bf16, fp32 accumulation): the fused routing kernel
+// k_prop (feature gather, three layers in LDS, top-k, candidate costing; one
+// launch per batch), the training GEMMs (k_gemm forward and data gradients,
+// k_gemm_tn split-K weight and bias gradients), softmax cross-entropy,
+// momentum SGD on fp32 master weights, and the feature / top-k kernels of the
+// general path (hidden widths other than 256, more than 64 candidates).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -431,8 +434,8 @@ __global__ __launch_bounds__(256) void k_lpat_topk(Geo g, const float* __restric
 // Fused proposal: features -> three MFMA layers -> top-k costing, per tile
 // ---------------------------------------------------------------------------
 
-// A workgroup of 256 threads takes TT (64 or 128) consecutive pairs (or input
-// rows).  Activations stay in LDS as one [TT][RS] bf16 buffer (features, then
+// A workgroup of 256 threads takes TT consecutive pairs (or input rows); 64
+// is used, so four workgroups fit a CU (128 VGPRs, 37 KB of LDS each).  Activations stay in LDS as one [TT][RS] bf16 buffer (features, then
 // H1, then H2), so only the grid cells, the weights and the results cross the
 // memory hierarchy.  Layers run transposed: for neurons n
 // and pairs r, D[n][r] = sum_k W^T[n][k] X[r][k] with W^T in MFMA fragment
@@ -803,7 +806,7 @@ PropFn prop_kernel_t(int Z) {
   }
 }
 template <bool ROUTE>
-PropFn prop_kernel(int Z, int tt) { return tt == 64 ? prop_kernel_t<ROUTE, 64>(Z) : prop_kernel_t<ROUTE, 128>(Z); }
+PropFn prop_kernel(int Z) { return prop_kernel_t<ROUTE, kTile>(Z); }
 
 int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
@@ -982,7 +985,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.ctr = c->d_ctr;
     A.stats = (unsigned long long*)c->d_prop_stats;
     A.fbout = c->collect ? c->d_pair_fb : nullptr;
-    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z, kTile), dim3(blocks_for(bm.P, kTile)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(blocks_for(bm.P, kTile)), dim3(256), 0, c->stream, A);
     HIPC(c, hipGetLastError());
     return 0;
   }
@@ -1058,7 +1061,7 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
       hipMemset(m->d_mom, 0, o * 4) != hipSuccess ||
       hipMemcpy(m->d_loglut, loglut.data(), kLut * 4, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(c, GR_ENODEV, "model upload failed"));
-  if (m->H == kHid && prop_kernel<true>(c->g.Z, kTile)) {
+  if (m->H == kHid && prop_kernel<true>(c->g.Z)) {
     m->nf3 = m->n_op / 16;  // every padded output column (the forward pass returns them all)
     if ((rc = dalloc(c, &m->W1p, (size_t)m->k_in * m->H)) || (rc = dalloc(c, &m->W2p, (size_t)m->H * m->H)) ||
         (rc = dalloc(c, &m->W3p, (size_t)m->H * m->n_op)))
@@ -1225,7 +1228,7 @@ int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
     A.X = m->a_X;
     A.P = n;
     A.logits = m->a_L;
-    hipLaunchKernelGGL(prop_kernel<false>(c->g.Z, kTile), dim3(blocks_for(n, kTile)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(prop_kernel<false>(c->g.Z), dim3(blocks_for(n, kTile)), dim3(256), 0, c->stream, A);
     HIPC(c, hipGetLastError());
   } else {
     forward(m, c->stream, n, m->a_X);
