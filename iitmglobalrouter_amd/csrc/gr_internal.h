@@ -96,6 +96,13 @@ struct gr_ctx {
   int prop_k = 8;
   float prop_thr = 0.6f;
   int64_t* d_prop_stats = nullptr;  // [pairs costed, pairs that fell back to full enumeration]
+  // while a fused proposal is set: every layer's remaining capacity, clamped
+  // to the cost table's range, per GCELL (16 int16 at (y*X + x)*16, layer z
+  // at +z), kept by the refresh on the cells it decrements; d_remz0 is the
+  // copy for the initial capacity (the reset's)
+  int16_t* d_remz = nullptr;
+  int16_t* d_remz0 = nullptr;
+  bool remz_live = false;
   // training-data capture after each batch's costing
   gr_mlp* collect = nullptr;
   int collect_per_batch = 0;

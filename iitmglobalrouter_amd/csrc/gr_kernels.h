@@ -52,6 +52,10 @@ namespace grd {
 // FLT: grids with fp32 capacities.  remf holds r per cell; a marked cell's r
 // is lowered by 1.0f in fp32 (bit_util.hpp:87, d_cube_data_bit[tid]--) and the
 // integer rem counts the commits; the cell cost is cost_f32(model, r).
+//
+// remz (integer grids, or null): the per-GCELL copy of every layer's clamped
+// remaining capacity that the learned proposal's lattice gather reads; a
+// marked cell's entry is rewritten with its new value.
 template <bool FLT>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  uint8_t* __restrict__ mark, float* __restrict__ pfx,
@@ -59,7 +63,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
                                                  float* __restrict__ remf, int model,
-                                                 float* __restrict__ vterm) {
+                                                 float* __restrict__ vterm, int16_t* __restrict__ remz) {
   extern __shared__ float sh[];
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
@@ -103,6 +107,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
             }
             vv = vv < kRMin ? kRMin : (vv > kRMin + kLut - 1 ? kRMin + kLut - 1 : vv);
             c = slut[vv - kRMin];
+            if (hit && remz) remz[((z & 1) ? (uint32_t)i * g.X + line : (uint32_t)line * g.X + i) * 16 + z] = (int16_t)vv;
           }
           if (hit) mark[base + i] = 0;
         }

@@ -430,6 +430,34 @@ __global__ __launch_bounds__(256) void k_lpat_topk(Geo g, const float* __restric
   }
 }
 
+// Per-GCELL copy of every layer's remaining capacity for the lattice gather:
+// dst[(y*X + x)*16 + z] = src at (z, x, y), clamped to the cost table's range
+// (z < Z; the rest 0).  One thread per GCELL, two 16-byte stores.
+__global__ __launch_bounds__(256) void k_remz_build(Geo g, const int32_t* __restrict__ src, int16_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)g.XY) return;
+  const int y = (int)(i / g.X), x = (int)(i - (int64_t)y * g.X);
+  uint32_t w[8];
+#pragma unroll
+  for (int h = 0; h < 8; ++h) {
+    uint32_t pk = 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int z = 2 * h + e;
+      int v = 0;
+      if (z < g.Z) {
+        v = src[enc(g, z, x, y)];
+        v = v < kRMin ? kRMin : (v > kRMin + kLut - 1 ? kRMin + kLut - 1 : v);
+      }
+      pk |= (uint32_t)(uint16_t)(int16_t)v << (16 * e);
+    }
+    w[h] = pk;
+  }
+  uint4* d = reinterpret_cast<uint4*>(dst + i * 16);
+  d[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  d[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 // ---------------------------------------------------------------------------
 // Fused proposal: features -> three MFMA layers -> top-k costing, per tile
 // ---------------------------------------------------------------------------
@@ -535,6 +563,7 @@ __device__ __forceinline__ void store_hidden(const v4f (&acc)[NF][PF], const flo
 struct PropArgs {
   Geo g;
   const int32_t* rem;
+  const int16_t* remz;  // per-GCELL layer copy of rem (k_remz_build; kept by the refresh)
   const float* loglut;
   const float* pfx;
   const float* vterm;
@@ -617,28 +646,33 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
     // all ZT layers of it (feature 4 + l*25 + u*5 + v), UG points in flight
     constexpr int NG = TT * 25, UG = 7;
     for (int gb = tid; gb < NG; gb += 256 * UG) {
-      uint32_t be[UG], bo[UG];
+      uint32_t xy[UG];
       int dst[UG];
 #pragma unroll
       for (int u = 0; u < UG; ++u) {
         const int gi0 = gb + 256 * u, gi = gi0 < NG ? gi0 : NG - 1;
         const int pr = gi / 25, uv = gi - 25 * pr, iu = uv / 5, iv = uv - 5 * iu;
-        const uint32_t xs = (uint32_t)sx[pr][iu], ys = (uint32_t)sy[pr][iv];
-        be[u] = ys * (uint32_t)g.X + xs;  // even layers: rows
-        bo[u] = xs * (uint32_t)g.Y + ys;  // odd layers: columns
+        xy[u] = (uint32_t)sy[pr][iv] * (uint32_t)g.X + (uint32_t)sx[pr][iu];
         dst[u] = gi0 < NG ? pr * RS + 4 + uv : -1;
       }
-      int32_t r[UG][ZT];
+      // all layers of a lattice point in one or two 16-byte loads (remz:
+      // values already clamped to the table's range)
+      uint4 q[UG][2];
 #pragma unroll
-      for (int u = 0; u < UG; ++u)
-#pragma unroll
-        for (int l = 0; l < ZT; ++l) r[u][l] = A.rem[(uint32_t)l * g.XY + ((l & 1) ? bo[u] : be[u])];
+      for (int u = 0; u < UG; ++u) {
+        const uint4* src = reinterpret_cast<const uint4*>(A.remz + (size_t)xy[u] * 16);
+        q[u][0] = src[0];
+        if (ZT > 8) q[u][1] = src[1];
+      }
 #pragma unroll
       for (int u = 0; u < UG; ++u) {
         if (dst[u] < 0) continue;
+        const uint32_t w[8] = {q[u][0].x, q[u][0].y, q[u][0].z, q[u][0].w,
+                               ZT > 8 ? q[u][1].x : 0u, ZT > 8 ? q[u][1].y : 0u, ZT > 8 ? q[u][1].z : 0u,
+                               ZT > 8 ? q[u][1].w : 0u};
 #pragma unroll
         for (int l = 0; l < ZT; ++l) {
-          const int v = r[u][l] < kRMin ? kRMin : (r[u][l] > kRMin + kLut - 1 ? kRMin + kLut - 1 : r[u][l]);
+          const int v = (int)(int16_t)(uint16_t)(w[l >> 1] >> (16 * (l & 1)));
           act[dst[u] + 25 * l] = slut[v - kRMin];
         }
       }
@@ -943,6 +977,7 @@ void mlp_detach_all(gr_ctx* c) {
   for (gr_mlp* m : c->models) m->ctx = nullptr;
   c->models.clear();
   c->prop = c->collect = nullptr;
+  c->remz_live = false;
 }
 
 // Exhaustive costing already ran; capture a strided sample of the batch's
@@ -969,6 +1004,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
   if (m->W1p) {
     PropArgs A = prop_args(m);
     A.rem = c->d_rem;
+    A.remz = c->d_remz;
     A.loglut = m->d_loglut;
     A.pfx = c->d_pfx;
     A.vterm = c->d_vterm;
@@ -1082,6 +1118,7 @@ void gr_mlp_destroy(gr_mlp* m) {
     (void)hipStreamSynchronize(c->stream);
     if (c->prop == m) {
       c->prop = nullptr;
+      c->remz_live = false;
       drop_graph(c);
     }
     if (c->collect == m) c->collect = nullptr;
@@ -1277,6 +1314,20 @@ int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   drop_graph(c);  // the captured pass holds the old costing kernel and its arguments
+  c->remz_live = false;
+  if (m && mlp_fused(m)) {  // the lattice gather's per-GCELL layer copy, from here on kept by the refresh
+    int rc;
+    if (!c->d_remz && ((rc = dalloc(c, &c->d_remz, (size_t)16 * c->g.XY)) ||
+                       (rc = dalloc(c, &c->d_remz0, (size_t)16 * c->g.XY))))
+      return rc;
+    hipLaunchKernelGGL(k_remz_build, dim3(blocks_for((int64_t)c->g.XY, 256)), dim3(256), 0, c->stream, c->g, c->d_cap0,
+                       c->d_remz0);
+    hipLaunchKernelGGL(k_remz_build, dim3(blocks_for((int64_t)c->g.XY, 256)), dim3(256), 0, c->stream, c->g, c->d_rem,
+                       c->d_remz);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->remz_live = true;
+  }
   c->prop = m;
   c->prop_k = k;
   c->prop_thr = threshold;

@@ -84,11 +84,11 @@ void launch_refresh(gr_ctx* c, int all) {
   if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->d_vterm);
+                       c->d_vterm, (int16_t*)nullptr);
   else
     hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->d_vterm);
+                       c->d_vterm, c->remz_live ? c->d_remz : (int16_t*)nullptr);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -432,6 +432,7 @@ void gr_destroy(gr_ctx* c) {
   (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
   (void)hipFree(c->d_pair_cost); (void)hipFree(c->d_pair_pat); (void)hipFree(c->d_pair_fb);
+  (void)hipFree(c->d_remz); (void)hipFree(c->d_remz0);
   free_sched(c->sched);
   free_sched(c->single);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -562,6 +563,9 @@ int gr_reset_demand(gr_ctx* c) {
   if (!c) return fail(c, GR_EINVAL, "null context");
   (void)hipSetDevice(c->dev);
   HIPC(c, hipMemcpyAsync(c->d_rem, c->d_cap0, sizeof(int32_t) * c->cs, hipMemcpyDeviceToDevice, c->stream));
+  if (c->remz_live)
+    HIPC(c, hipMemcpyAsync(c->d_remz, c->d_remz0, sizeof(int16_t) * 16 * c->g.XY, hipMemcpyDeviceToDevice,
+                           c->stream));
   if (c->d_remf)
     HIPC(c, hipMemcpyAsync(c->d_remf, c->d_capf0, sizeof(float) * c->cs, hipMemcpyDeviceToDevice, c->stream));
   return refresh_all(c);

@@ -103,7 +103,9 @@ def test_topk_choice_matches_oracle(gr, orc, gpu, Z):
     k = {4: 2, 10: 3, 11: 8, 12: 5}[Z]
     r.set_proposal(m, k=k, threshold=0.0)
     Z2 = (Z + 1) // 2
-    for b in range(2):
+    # several batches: the later ones gather their features from the per-cell
+    # copy the refresh keeps, which must equal the state the oracle sees
+    for b in range(4):
         batch = ids[ofst[b]:ofst[b + 1]]
         src, dst = [], []
         for nid in batch:
