@@ -462,20 +462,22 @@ __global__ __launch_bounds__(256) void k_remz_build(Geo g, const int32_t* __rest
 // Fused proposal: features -> three MFMA layers -> top-k costing, per tile
 // ---------------------------------------------------------------------------
 
-// A workgroup of 256 threads takes TT consecutive pairs (or input rows); 64
-// is used, so four workgroups fit a CU (128 VGPRs, 37 KB of LDS each).  Activations stay in LDS as one [TT][RS] bf16 buffer (features, then
+// A workgroup of NW = TT/16 wavefronts takes TT consecutive pairs (or input
+// rows); TT = 128 is used: two 8-wave workgroups per CU (at most 128 VGPRs,
+// 74 KB of LDS each), and each weight fragment serves 128 pairs.  Activations stay in LDS as one [TT][RS] bf16 buffer (features, then
 // H1, then H2), so only the grid cells, the weights and the results cross the
 // memory hierarchy.  Layers run transposed: for neurons n
 // and pairs r, D[n][r] = sum_k W^T[n][k] X[r][k] with W^T in MFMA fragment
 // order (see k_pack) as the A operand from global memory (L2-resident, 280 KB
 // at the 256-wide model) and X from LDS as the B operand, so each lane's four
 // results are four consecutive neurons of one pair: one 8-byte LDS store.
-// Hidden layers: wave w owns neurons [64w, 64w + 64) for all TT pairs
-// (4 x TT/16 fragments).  Output layer: wave w owns pairs [TT/4 w, TT/4 (w + 1))
-// for all output neurons.  The K order of every sum and
+// Hidden layers: wave w owns neurons [16 NFH w, 16 NFH (w + 1)) for all TT
+// pairs (NFH = 16/NW fragments of 16 neurons).  Output layer: wave f owns the
+// output fragment f for all pairs; the logits then go through LDS to the
+// top-k, where wave w takes pairs [16w, 16w + 16).  The K order of every sum and
 // the bias, ReLU and bf16 rounding points are those of k_gemm's forward pass.
 constexpr int kHid = 256;  // hidden width served by the fused kernel
-constexpr int kTile = 64;  // rows per workgroup of the fused kernel (four fit a CU)
+constexpr int kTile = 128;  // rows per workgroup of the fused kernel (8 wavefronts; two fit a CU)
 
 template <int ZT>
 struct PropShape {
@@ -591,10 +593,11 @@ struct PropArgs {
 // from the output fragments, then candidate costing with the full-enumeration
 // fallback (k_lpat_topk's rules).  !ROUTE: rows of A.X in, logits out (the
 // inference forward pass).
-template <int ZT, bool ROUTE, int TT>
-__global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
-  static_assert(TT == 64 || TT == 128, "tile of 64 or 128 rows");
-  constexpr int QW = TT / 64;  // output-layer pair fragments per wave
+template <int ZT, bool ROUTE, int TT, int NW = TT / 16>
+__global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
+  constexpr int NT = 64 * NW;
+  constexpr int NFH = 16 / NW;
+  constexpr int QW = TT / 16 / NW;
   using S = PropShape<ZT>;
   constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, NF3 = S::NF3, NOUT = S::NOUT;
   __shared__ __attribute__((aligned(16))) uint16_t act[TT * RS];
@@ -608,15 +611,15 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
   const int64_t t0 = (int64_t)blockIdx.x * TT;
   const int nrow = (int)(A.P - t0 < TT ? A.P - t0 : TT);
   const Geo& g = A.g;
-  const uint16_t* w1 = A.W1p + (size_t)wv * 4 * (KIN / 32) * 512;
-  const uint16_t* w2 = A.W2p + (size_t)wv * 4 * (kHid / 32) * 512;
-  uint4 a0[4];  // first-step weight fragments of the next layer, in flight across the phase before it
-  mma_first<KIN / 32, 4>(w1, a0, lane);
+  const uint16_t* w1 = A.W1p + (size_t)wv * NFH * (KIN / 32) * 512;
+  const uint16_t* w2 = A.W2p + (size_t)wv * NFH * (kHid / 32) * 512;
+  uint4 a0[NFH];
+  mma_first<KIN / 32, NFH>(w1, a0, lane);
 
   if (ROUTE) {
     if (blockIdx.x == 0 && tid == 0) *A.ctr += 1;  // this batch's stamp
     if (tid == 0) nfb = 0;
-    for (int i = tid; i < kLut; i += 256) slut[i] = f2bf(A.loglut[i]);
+    for (int i = tid; i < kLut; i += NT) slut[i] = f2bf(A.loglut[i]);
     if (tid < TT) {  // waves 0 and 1 whole: the owner search is wave-collective
       const int64_t w = t0 + (tid < nrow ? tid : nrow - 1);
       const int pos = pair_owner_wave(A.bpair, A.chunk_net, A.n, w);
@@ -645,12 +648,12 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
     // lattice features: one lattice point (pair, u, v) per thread and step,
     // all ZT layers of it (feature 4 + l*25 + u*5 + v), UG points in flight
     constexpr int NG = TT * 25, UG = 7;
-    for (int gb = tid; gb < NG; gb += 256 * UG) {
+    for (int gb = tid; gb < NG; gb += NT * UG) {
       uint32_t xy[UG];
       int dst[UG];
 #pragma unroll
       for (int u = 0; u < UG; ++u) {
-        const int gi0 = gb + 256 * u, gi = gi0 < NG ? gi0 : NG - 1;
+        const int gi0 = gb + NT * u, gi = gi0 < NG ? gi0 : NG - 1;
         const int pr = gi / 25, uv = gi - 25 * pr, iu = uv / 5, iv = uv - 5 * iu;
         xy[u] = (uint32_t)sy[pr][iv] * (uint32_t)g.X + (uint32_t)sx[pr][iu];
         dst[u] = gi0 < NG ? pr * RS + 4 + uv : -1;
@@ -679,7 +682,7 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
     }
   } else {
     constexpr int C8 = KIN / 8;
-    for (int i = tid; i < TT * C8; i += 256) {
+    for (int i = tid; i < TT * C8; i += NT) {
       const int pr = i / C8, c = i - pr * C8;
       const int64_t row = t0 + (pr < nrow ? pr : nrow - 1);
       *reinterpret_cast<uint4*>(act + pr * RS + c * 8) =
@@ -690,40 +693,60 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
 
   // hidden layers
   {
-    v4f acc[4][TT / 16];
-    mma_tile<KIN / 32, 4, TT / 16>(w1, a0, act, RS, 0, acc, lane);
-    mma_first<kHid / 32, 4>(w2, a0, lane);
+    v4f acc[NFH][TT / 16];
+    mma_tile<KIN / 32, NFH, TT / 16>(w1, a0, act, RS, 0, acc, lane);
+    mma_first<kHid / 32, NFH>(w2, a0, lane);
     __syncthreads();
-    store_hidden<4, TT / 16>(acc, A.b1, wv * 64, act, RS, lane);
+    store_hidden<NFH, TT / 16>(acc, A.b1, wv * 16 * NFH, act, RS, lane);
     __syncthreads();
-    mma_tile<kHid / 32, 4, TT / 16>(w2, a0, act, RS, 0, acc, lane);
+    mma_tile<kHid / 32, NFH, TT / 16>(w2, a0, act, RS, 0, acc, lane);
     __syncthreads();
-    store_hidden<4, TT / 16>(acc, A.b2, wv * 64, act, RS, lane);
+    store_hidden<NFH, TT / 16>(acc, A.b2, wv * 16 * NFH, act, RS, lane);
     __syncthreads();
   }
 
-  // output layer: lane l holds, for pair 32wv + 16q + (l & 15), the logits of
-  // output neurons 16f + 4(l >> 4) + i (f < NFL, i < 4)
+  // output layer, split by neuron fragment: wave f < NFL computes output
+  // neurons [16f, 16f + 16) for all TT pairs, so each W3 fragment is loaded
+  // once per tile; lane l then holds, for pair 16q + (l & 15), the logits of
+  // neurons 16f + 4(l >> 4) + i (i < 4)
   constexpr int NFL = ROUTE ? NF3 : S::NFO;
-  v4f acc[NFL][QW];
-  uint4 a3[NFL];
-  mma_first<kHid / 32, NFL>(A.W3p, a3, lane);
-  mma_tile<kHid / 32, NFL, QW>(A.W3p, a3, act, RS, 16 * QW * wv, acc, lane);
+  static_assert(NFL <= NW, "one output fragment per wave");
+  const bool has3 = wv < NFL;  // wave-uniform
+  v4f acc[1][TT / 16];
+  float4 b3v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (has3) {
+    const uint16_t* w3 = A.W3p + (size_t)wv * (kHid / 32) * 512;
+    uint4 a3[1];
+    mma_first<kHid / 32, 1>(w3, a3, lane);
+    mma_tile<kHid / 32, 1, TT / 16>(w3, a3, act, RS, 0, acc, lane);
+    b3v = *reinterpret_cast<const float4*>(A.b3 + 16 * wv + 4 * (lane >> 4));
+  }
   if (!ROUTE) {
+    if (has3) {
+      const int n = 16 * wv + 4 * (lane >> 4);
 #pragma unroll
-    for (int f = 0; f < NFL; ++f) {
-      const int n = 16 * f + 4 * (lane >> 4);
-      const float4 bb = *reinterpret_cast<const float4*>(A.b3 + n);
-#pragma unroll
-      for (int q = 0; q < QW; ++q) {
-        const int pr = 16 * QW * wv + 16 * q + (lane & 15);
+      for (int q = 0; q < TT / 16; ++q) {
+        const int pr = 16 * q + (lane & 15);
         if (pr < nrow)
           *reinterpret_cast<float4*>(A.logits + (t0 + pr) * A.n_op + n) =
-              make_float4(acc[f][q][0] + bb.x, acc[f][q][1] + bb.y, acc[f][q][2] + bb.z, acc[f][q][3] + bb.w);
+              make_float4(acc[0][q][0] + b3v.x, acc[0][q][1] + b3v.y, acc[0][q][2] + b3v.z, acc[0][q][3] + b3v.w);
       }
     }
     return;
   }
+  // logits (fp32) into LDS over the activations, once every wave is done with them
+  constexpr int LS = NF3 * 16 + 4;  // row stride in floats
+  static_assert(TT * LS * 4 <= TT * RS * 2, "logits fit the activation buffer");
+  float* lg = reinterpret_cast<float*>(act);
+  __syncthreads();
+  if (has3) {
+    const int n = 16 * wv + 4 * (lane >> 4);
+#pragma unroll
+    for (int q = 0; q < TT / 16; ++q)
+      *reinterpret_cast<float4*>(lg + (16 * q + (lane & 15)) * LS + n) =
+          make_float4(acc[0][q][0] + b3v.x, acc[0][q][1] + b3v.y, acc[0][q][2] + b3v.z, acc[0][q][3] + b3v.w);
+  }
+  __syncthreads();
 
   // top-k per pair over its 4 lanes: k rounds, each taking the best (value
   // desc, index asc) logit not yet taken; softmax top-1 probability from the
@@ -733,14 +756,15 @@ __global__ __launch_bounds__(256, TT == 64 ? 4 : 2) void k_prop(PropArgs A) {
   for (int q = 0; q < QW; ++q) {
     float vals[NF3][4];
     float mxl = -INFINITY;
+    const int prq = 16 * QW * wv + 16 * q + (lane & 15);
 #pragma unroll
     for (int f = 0; f < NF3; ++f) {
       const int nb = 16 * f + 4 * gq;
-      const float4 bb = *reinterpret_cast<const float4*>(A.b3 + nb);
-      vals[f][0] = acc[f][q][0] + bb.x;
-      vals[f][1] = acc[f][q][1] + bb.y;
-      vals[f][2] = acc[f][q][2] + bb.z;
-      vals[f][3] = acc[f][q][3] + bb.w;
+      const float4 lv4 = *reinterpret_cast<const float4*>(lg + prq * LS + nb);
+      vals[f][0] = lv4.x;
+      vals[f][1] = lv4.y;
+      vals[f][2] = lv4.z;
+      vals[f][3] = lv4.w;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         if (nb + i < NOUT) mxl = fmaxf(mxl, vals[f][i]);
@@ -1021,7 +1045,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.ctr = c->d_ctr;
     A.stats = (unsigned long long*)c->d_prop_stats;
     A.fbout = c->collect ? c->d_pair_fb : nullptr;
-    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(blocks_for(bm.P, kTile)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(blocks_for(bm.P, kTile)), dim3(kTile * 4), 0, c->stream, A);
     HIPC(c, hipGetLastError());
     return 0;
   }
@@ -1265,7 +1289,7 @@ int gr_mlp_forward(gr_mlp* m, const float* x, int32_t n, float* logits) {
     A.X = m->a_X;
     A.P = n;
     A.logits = m->a_L;
-    hipLaunchKernelGGL(prop_kernel<false>(c->g.Z), dim3(blocks_for(n, kTile)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(prop_kernel<false>(c->g.Z), dim3(blocks_for(n, kTile)), dim3(kTile * 4), 0, c->stream, A);
     HIPC(c, hipGetLastError());
   } else {
     forward(m, c->stream, n, m->a_X);
