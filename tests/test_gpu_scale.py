@@ -46,11 +46,11 @@ def test_huge_nets(gr, orc, gpu):
     assert np.array_equal(r.demand(), o.demand())
 
 
-@pytest.mark.parametrize("kind", [3, 5])
-def test_2000_sampled_batches(gr, orc, gpu, kind):
-    X, Y, Z = 2000, 2000, 10
+@pytest.mark.parametrize("kind,X,nets", [(2, 800, 150000), (3, 2000, 1500000), (5, 2000, 1500000)])
+def test_full_workload_sampled_batches(gr, orc, gpu, kind, X, nets):
+    Y, Z = X, 10
     cap = gr.gen_grid(kind, 0, X, Y, Z)
-    pfx, pins = gr.gen_nets(kind, 0, X, Y, Z, 1500000)
+    pfx, pins = gr.gen_nets(kind, 0, X, Y, Z, nets)
     ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
     nb = len(ofst) - 1
     deg = np.diff(pfx)
@@ -85,5 +85,17 @@ def test_2000_sampled_batches(gr, orc, gpu, kind):
     last = slot0[1:] - 1
     assert np.all(e[last] == -1) and np.all(l[last] == -1)
     assert int((e >= 0).sum()) == int((deg - 1).sum())
+    # every net's edges connect all of its pins: over the whole pass the edge
+    # graph (node = schedule slot) has exactly one component per net
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    pos = np.repeat(np.arange(len(ids)), deg[ids])  # schedule position of every slot
+    ok = e >= 0
+    src = slot0[pos[ok]] + ((e[ok] >> 15) & 0x7FFF)
+    dst = slot0[pos[ok]] + (e[ok] & 0x7FFF)
+    assert np.all((e[ok] >> 30) == np.arange(len(ids))[pos[ok]] - ofst[np.searchsorted(ofst, pos[ok], "right") - 1])
+    n = int(slot0[-1])
+    g = coo_matrix((np.ones(len(src), np.int8), (src, dst)), shape=(n, n))
+    assert connected_components(g, directed=False)[0] == len(ids)
     if kind == 5:  # the stress grid really overflows
         assert int(np.maximum(r.demand() - cap, 0).sum()) > 0
