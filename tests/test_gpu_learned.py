@@ -258,3 +258,34 @@ def test_capture_under_proposal_keeps_fallback_labels(gr, gpu):
         with pytest.raises(gr.RouterError):
             mb.train(epochs=1, batch=32, lr=0.05, seed=1)
     b.set_proposal(None)
+
+
+def test_always_fallback_full_workload(gr, gpu):
+    """800x800x10, 150,000 nets requested, whole passes as graph replays:
+    through the fused kernel with every pair falling back (threshold 2) the
+    routes and demand equal plain routing bit for bit, on every replay."""
+    X, Y, Z = 800, 800, 10
+    cap = gr.gen_grid(2, 0, X, Y, Z)
+    pfx, pins = gr.gen_nets(2, 0, X, Y, Z, 150000)
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    r.route_schedule()
+    r.sync()
+    e0, l0, c0 = r.schedule_outputs()
+    d0 = r.demand()
+    m = gr.Model(r, seed=1)
+    r.set_proposal(m, k=4, threshold=2.0)
+    r.proposal_stats()
+    for rep in range(2):
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+        e, l, c = r.schedule_outputs()
+        assert np.array_equal(e, e0) and np.array_equal(l, l0), rep
+        assert np.array_equal(np.asarray(c).view(np.int32), np.asarray(c0).view(np.int32)), rep
+        assert np.array_equal(r.demand(), d0), rep
+    pairs, fb = r.proposal_stats()
+    assert pairs == 2 * int((np.diff(pfx) * (np.diff(pfx) - 1) // 2).sum()) and fb == pairs
+    r.set_proposal(None)
