@@ -386,6 +386,18 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = n_nets / ((t1 - t0) / args.steps)
     print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    # the same passes launched kernel by kernel (no graph): the per-kernel-launch
+    # form the captured graph replaces (a pass split in two ranges is not replayed)
+    nb_all = len(ofst) - 1
+    r.sync()
+    te = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset_demand()
+        r.route_schedule(0, nb_all - 1)
+        r.route_schedule(nb_all - 1, 1)
+    r.sync()
+    eager_ms = (time.perf_counter() - te) * 1e3 / args.steps
+    print(f"# per-kernel launches: {eager_ms:.3f} ms/step", file=sys.stderr, flush=True)
 
     deg, P, F = footprint(X, Y, Z, pfx, pins)
     G = int(r.demand().sum())  # committed cells over the pass = total demand bumps
@@ -465,6 +477,8 @@ def main():
         "stress": stress,
         "true_mst": true_mst,
         "callers": io,
+        "per_kernel_launches": {"ms_per_step": round(eager_ms, 4), "nets_per_s": round(n_nets / (eager_ms / 1e3), 1),
+                                "graph_speedup": round(eager_ms / ms_step, 3)},
         "path_bytes_per_step": bytes_path,
         "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
     }
