@@ -1072,6 +1072,8 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
   *out = nullptr;
   if (c->d_remf) return fail(c, GR_ESTATE, "the learned proposal needs a grid with integer capacities");
   if ((c->g.Z - 1) * c->g.Z2 > 128) return fail(c, GR_EINVAL, "the learned proposal scores at most 128 patterns (Z <= 16)");
+  if (c->g.X > 32767 || c->g.Y > 32767)  // pair coordinates and lattice points are int16 in LDS
+    return fail(c, GR_EINVAL, "the learned proposal supports grids of at most 32767 GCELLs per side");
   (void)hipSetDevice(c->dev);
   gr_mlp* m = new gr_mlp();
   m->ctx = c;
