@@ -8,7 +8,7 @@ namespace grd {
 constexpr int kRMin = -256;
 constexpr int kLut = 512;  // r in [-256, 255]
 constexpr int kSmallDeg = 64;  // nets up to this many pins: one wavefront per net
-constexpr int kLargeDeg = 2048;  // LDS-resident tree for nets up to this many pins
+constexpr int kLargeDeg = 4096;  // LDS-resident tree (36 B per pin) for nets up to this many pins
 
 struct Geo {
   int X, Y, Z, Z2;

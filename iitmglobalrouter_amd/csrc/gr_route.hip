@@ -122,7 +122,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       const int32_t id = net_ids[m.begin + i];
       if (id < 0 || id >= c->n_nets) return fail(c, GR_EINVAL, "net id out of range");
       const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
-      if (d > kLargeDeg) return fail(c, GR_ERANGE, "net has more pins than the device tree supports (2048)");
+      if (d > kLargeDeg)
+        return fail(c, GR_ERANGE, "net has more pins than the device tree supports (" + std::to_string(kLargeDeg) + ")");
       slot[m.begin + i] = slots;
       pair[m.begin + i] = P;
       pinofs[m.begin + i] = c->h_pin_pfx[id];

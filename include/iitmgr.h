@@ -129,7 +129,9 @@ int gr_route_batch(gr_ctx* ctx, const int32_t* net_ids, int32_t n,
 /*
  * Keep a whole batch schedule resident on the device: batch b covers
  * net_ids[batch_ofst[b] .. batch_ofst[b+1]).  Replaces the batcher's per-batch
- * uploads and scans (part_net/net_part.hpp:345-369).
+ * uploads and scans (part_net/net_part.hpp:345-369).  A routed net may have
+ * at most 4096 pins (its tree is built in one workgroup's LDS); larger nets
+ * give GR_ERANGE here and in gr_route_batch.
  */
 int gr_set_schedule(gr_ctx* ctx, int32_t n_batches, const int32_t* batch_ofst,
                     const int32_t* net_ids);

@@ -23,7 +23,7 @@ def test_huge_nets(gr, orc, gpu):
     cap = gr.gen_grid(2, 9, X, Y, Z)
     rng = np.random.default_rng(9)
     nets = []
-    for npins in (900, 620, 333, 170, 90, 40, 7, 2):
+    for npins in (2500, 900, 620, 333, 170, 90, 40, 7, 2):
         s = set()
         while len(s) < npins:
             s.add(_enc(X, Y, int(rng.integers(0, 2)), int(rng.integers(0, X)), int(rng.integers(0, Y))))
