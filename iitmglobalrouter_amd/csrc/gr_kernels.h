@@ -690,32 +690,36 @@ struct SmallClasses {
 };
 
 template <int G>
-__device__ void run_group_class(const TreeArgs& A0, const SmallClasses& C, int cls, GroupLds* lds) {
+__device__ void run_group_class(const TreeArgs& A0, const SmallClasses& C, int cls, int b, int W, GroupLds* lds) {
   TreeArgs A = A0;
   A.list = C.list[cls];
   A.n_list = C.n[cls];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = ((blockIdx.x - C.block0[cls]) * 4 + wv) * (64 / G) + lane / G;
+  const int idx = ((b - C.block0[cls]) * W + wv) * (64 / G) + lane / G;
   tree_group<G>(A, idx, lds[wv]);
+}
+
+// Small-net work of block b (block0 ranges of C): W wavefronts, lds[W].
+__device__ void small_block(const TreeArgs& A, const SmallClasses& C, int b, int W, GroupLds* lds) {
+  if (b < C.block0[1]) {
+    const int idx = b * 64 * W + threadIdx.x;
+    if (idx < C.n[0]) tree2_commit(A, C.list[0][idx]);
+  } else if (b < C.block0[2]) {
+    run_group_class<8>(A, C, 1, b, W, lds);
+  } else if (b < C.block0[3]) {
+    run_group_class<16>(A, C, 2, b, W, lds);
+  } else if (b < C.block0[4]) {
+    run_group_class<32>(A, C, 3, b, W, lds);
+  } else {
+    run_group_class<64>(A, C, 4, b, W, lds);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C) {
   TreeArgs A = A0;
   A.bstamp = *A0.ctr;
   __shared__ GroupLds lds[4];
-  const int b = blockIdx.x;
-  if (b < C.block0[1]) {
-    const int idx = b * 256 + threadIdx.x;
-    if (idx < C.n[0]) tree2_commit(A, C.list[0][idx]);
-  } else if (b < C.block0[2]) {
-    run_group_class<8>(A, C, 1, lds);
-  } else if (b < C.block0[3]) {
-    run_group_class<16>(A, C, 2, lds);
-  } else if (b < C.block0[4]) {
-    run_group_class<32>(A, C, 3, lds);
-  } else {
-    run_group_class<64>(A, C, 4, lds);
-  }
+  small_block(A, C, blockIdx.x, 4, lds);
 }
 
 // One workgroup per net of more than kSmallDeg pins; LDS sized by the largest
@@ -736,6 +740,36 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int s
   float* scost = ecost + cap;  // scap floats
   tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, link2, edges, ecost, wcnt,
               s_sh, scost, scap);
+}
+
+// Small and large nets of one batch in one launch (no second stream, so no
+// fork and join between the costing and the refresh): workgroups
+// [0, n_large) take one large net each with all blockDim threads, the others
+// run small-net work on their first W wavefronts (the rest exit at once).
+// The dynamic LDS holds either the large tree (cap nodes, scap keys) or W
+// group areas; the launch takes the larger of the two.
+__global__ __launch_bounds__(1024) void k_tree_mixed(TreeArgs A0, SmallClasses C, int n_large, int cap, int scap,
+                                                     int W) {
+  TreeArgs A = A0;
+  A.bstamp = *A0.ctr;
+  extern __shared__ unsigned long long shl[];
+  __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
+  __shared__ int s_sh[2];
+  if ((int)blockIdx.x < n_large) {
+    unsigned long long* best = shl;
+    int2* edges = (int2*)(best + cap);
+    int* root = (int*)(edges + cap);
+    int* partner = root + cap;
+    int* link = partner + cap;
+    int* link2 = link + cap;
+    float* ecost = (float*)(link2 + cap);
+    float* scost = ecost + cap;
+    tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, link2, edges, ecost, wcnt,
+                s_sh, scost, scap);
+  } else {
+    if ((int)(threadIdx.x >> 6) >= W) return;
+    small_block(A, C, (int)blockIdx.x - n_large, W, (GroupLds*)shl);
+  }
 }
 
 }  // namespace grd

@@ -270,6 +270,42 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       blocks += (C.n[k] + per_block - 1) / per_block;
     }
     C.block0[5] = blocks;
+    const int cap = m.max_large;
+    const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
+    const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
+    const size_t large_lds = (size_t)cap * 36 + (size_t)scap * 4;
+    if (c->tune.fork >= 2 && blocks && m.cls_n[kClsLarge]) {
+      // one launch for both kinds when two of its workgroups fit a CU's LDS
+      // (the small-net blocks then run W wavefronts each, as four 4-wavefront
+      // blocks would); with fork = 3 also at one workgroup per CU, the
+      // small-net blocks using as many group areas as the large tree's LDS
+      // holds; otherwise the second-stream branch below
+      const int Wmax = c->tune.large_threads / 64;
+      int W = std::min(8, Wmax);
+      size_t lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+      bool one = 2 * (lds + 1024) <= (size_t)c->lds_max;
+      if (!one && c->tune.fork == 3) {
+        W = (int)std::min<size_t>((size_t)Wmax, std::max<size_t>(8, large_lds / sizeof(GroupLds)));
+        lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+        one = lds + 1024 <= (size_t)c->lds_max;
+      }
+      if (one) {
+        SmallClasses Cm = C;
+        int mb = 0;
+        for (int k = 0; k < 5; ++k) {
+          Cm.block0[k] = mb;
+          const int per_block = k == 0 ? 64 * W : 64 * W / kClsMax[k];
+          mb += (Cm.n[k] + per_block - 1) / per_block;
+        }
+        Cm.block0[5] = mb;
+        A.list = s.d_list + m.cls_ofs[kClsLarge];
+        A.n_list = m.cls_n[kClsLarge];
+        hipLaunchKernelGGL(k_tree_mixed, dim3(A.n_list + mb), dim3(c->tune.large_threads), lds, c->stream, A, Cm,
+                           A.n_list, cap, scap, W);
+        goto trees_done;
+      }
+    }
+    {
     // small and large nets are independent: with both present, the large-net
     // workgroups run on a second stream (a parallel branch of the graph)
     const bool fork = c->tune.fork && blocks && m.cls_n[kClsLarge] && !c->timing;
@@ -283,16 +319,14 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     if (m.cls_n[kClsLarge]) {
       A.list = s.d_list + m.cls_ofs[kClsLarge];
       A.n_list = m.cls_n[kClsLarge];
-      const int cap = m.max_large;
-      const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
-      const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
-      const size_t lds = (size_t)cap * 36 + (size_t)scap * 4;
-      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(c->tune.large_threads), lds, lst, A, cap, scap);
+      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(c->tune.large_threads), large_lds, lst, A, cap, scap);
     }
     if (fork) {
       HIPC(c, hipEventRecord(c->ev_join, c->stream2));
       HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
+    }
+  trees_done:;
   }
   {
     KTimer t(c, GR_K_REFRESH);
@@ -393,6 +427,8 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   }
   if (!ok ||
       hipFuncSetAttribute((const void*)k_tree_large, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)c->lds_max - 2048) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tree_mixed, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
@@ -624,13 +660,14 @@ int gr_get_tuning(const gr_ctx* c, gr_tuning* t) {
 int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
   if (!c || !t) return fail(c, GR_EINVAL, "null argument");
   if ((t->large_threads != 256 && t->large_threads != 512 && t->large_threads != 1024) ||
-      (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || (t->lpat_wave != 0 && t->lpat_wave != 1))
+      (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || t->fork < 0 || t->fork > 3 ||
+      (t->lpat_wave != 0 && t->lpat_wave != 1))
     return fail(c, GR_EINVAL, "tuning out of range");
   (void)hipSetDevice(c->dev);
   HIPC(c, hipStreamSynchronize(c->stream));
   const bool reclass = t->group_max != c->tune.group_max;
   c->tune = *t;
-  c->tune.fork = t->fork ? 1 : 0;
+  c->tune.fork = t->fork;
   Sched& s = c->sched;
   if (s.exec) { (void)hipGraphExecDestroy(s.exec); s.exec = nullptr; }
   s.graph_failed = false;
@@ -669,7 +706,7 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
   int rc = 0;
   for (int gm : {64, 32})
     for (int lt : {1024, 512, 256})
-      for (int fk : {1, 0})
+      for (int fk : {3, 1, 0})
         for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
             if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z

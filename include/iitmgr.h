@@ -191,10 +191,14 @@ int gr_lut_size(void);
 
 /*
  * Schedule parameters.  They change speed, never results (every choice is
- * bit-identical).  Defaults: {1024, 64, 1, 2048, 0}.
+ * bit-identical).  Defaults: {1024, 64, 3, 2048, 0}.
  *   large_threads   workgroup size for nets above group_max pins (256, 512 or 1024)
  *   group_max       largest pin count handled by a wavefront group of <= 64 lanes (32 or 64)
- *   fork            1: large-net trees on a second stream, parallel to the small ones
+ *   fork            how a batch with small and large nets runs its trees:
+ *                   0: two launches in sequence; 1: the large-net launch on a second
+ *                   stream, parallel to the small-net one; 2: one launch for both when
+ *                   two of its workgroups fit a CU's LDS, else as 1; 3: one launch
+ *                   for both whenever the large tree fits in LDS
  *   refresh_blocks  persistent grid of the cost refresh (>= 1)
  *   lpat_wave       1: one wavefront per pin pair instead of one lane per pair
  */

@@ -117,7 +117,10 @@ def test_tunings_are_bit_identical(gr, gpu):
     r.set_schedule(ofst, ids)
     ref = None
     for t in ({}, dict(large_threads=256, group_max=32), dict(large_threads=512, fork=0, refresh_blocks=333),
-              dict(lpat_wave=1, group_max=64, large_threads=1024, fork=1)):
+              dict(lpat_wave=1, group_max=64, large_threads=1024, fork=1), dict(fork=2),
+              dict(fork=2, large_threads=512, group_max=32), dict(fork=2, large_threads=256, group_max=64),
+              dict(fork=3, large_threads=1024, group_max=32), dict(fork=3, large_threads=256, group_max=32),
+              dict(fork=1, large_threads=1024, group_max=64)):
         r.set_tuning(**t)
         r.reset_demand()
         r.route_schedule()
