@@ -15,8 +15,9 @@
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
 // Per batch:
 //   k_lpat_lane  one lane per pin pair evaluates every L-pattern serially
-//   k_tree_small/_large  root-pair Boruvka + commit: one lane per 2-pin net,
-//                one wavefront per net up to 64 pins, one workgroup above
+//   k_tree_small/_large/_mixed  root-pair Boruvka + commit: one lane per
+//                2-pin net, one wavefront per net up to 64 pins, one workgroup
+//                above; _mixed runs both kinds of one batch in one launch
 //   k_refresh    per dirty run: demand -1 on marked cells, canonical Fenwick
 //                nodes rebuilt in LDS, run's pfx rewritten
 #pragma once
