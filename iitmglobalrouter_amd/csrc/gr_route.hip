@@ -214,6 +214,73 @@ LpatFn lane_kernel(int Z) {
   }
 }
 
+// Tree stage of one batch.  Small nets (class lists 0..4) and large nets are
+// independent.  With both present, tuning fork selects: 0 two launches in
+// sequence; 1 the large-net launch on a second stream (a parallel branch of
+// the graph, joined before the refresh); 2 one k_tree_mixed launch when two
+// of its workgroups fit a CU's LDS (the small-net blocks then run W = 8
+// wavefronts each, as two 4-wavefront blocks would), else as 1; 3 also one
+// mixed launch at one workgroup per CU, the small-net blocks using as many
+// group areas as the large tree's LDS already holds.
+int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
+  SmallClasses C;
+  int blocks = 0;
+  for (int k = 0; k < 5; ++k) {
+    C.list[k] = s.d_list + m.cls_ofs[k];
+    C.n[k] = m.cls_n[k];
+    C.block0[k] = blocks;
+    const int per_block = k == 0 ? 256 : 256 / kClsMax[k];
+    blocks += (C.n[k] + per_block - 1) / per_block;
+  }
+  C.block0[5] = blocks;
+  const int n_large = m.cls_n[kClsLarge];
+  const int cap = m.max_large;
+  const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
+  const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
+  const size_t large_lds = (size_t)cap * 36 + (size_t)scap * 4;
+  TreeArgs L = A;
+  L.list = s.d_list + m.cls_ofs[kClsLarge];
+  L.n_list = n_large;
+  if (c->tune.fork >= 2 && blocks && n_large) {
+    const int Wmax = c->tune.large_threads / 64;
+    int W = std::min(8, Wmax);
+    size_t lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+    bool fits = 2 * (lds + 1024) <= (size_t)c->lds_max;
+    if (!fits && c->tune.fork == 3) {
+      W = (int)std::min<size_t>((size_t)Wmax, std::max<size_t>(8, large_lds / sizeof(GroupLds)));
+      lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+      fits = lds + 1024 <= (size_t)c->lds_max;
+    }
+    if (fits) {
+      SmallClasses Cm = C;
+      int mb = 0;
+      for (int k = 0; k < 5; ++k) {
+        Cm.block0[k] = mb;
+        const int per_block = k == 0 ? 64 * W : 64 * W / kClsMax[k];
+        mb += (Cm.n[k] + per_block - 1) / per_block;
+      }
+      Cm.block0[5] = mb;
+      hipLaunchKernelGGL(k_tree_mixed, dim3(n_large + mb), dim3(c->tune.large_threads), lds, c->stream, L, Cm,
+                         n_large, cap, scap, W);
+      return 0;
+    }
+  }
+  const bool fork = c->tune.fork && blocks && n_large && !c->timing;
+  hipStream_t lst = c->stream;
+  if (fork) {
+    HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPC(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    lst = c->stream2;
+  }
+  if (blocks) hipLaunchKernelGGL(k_tree_small, dim3(blocks), dim3(256), 0, c->stream, A, C);
+  if (n_large) hipLaunchKernelGGL(k_tree_large, dim3(n_large), dim3(c->tune.large_threads), large_lds, lst, L, cap, scap);
+  if (fork) {
+    HIPC(c, hipEventRecord(c->ev_join, c->stream2));
+    HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
+  return 0;
+}
+
 int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   const BatchMeta& m = s.meta[b];
   if (m.n == 0) return 0;
@@ -260,73 +327,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.true_mst = c->true_mst;
   {
     KTimer t(c, GR_K_TREE);
-    SmallClasses C;
-    int blocks = 0;
-    for (int k = 0; k < 5; ++k) {
-      C.list[k] = s.d_list + m.cls_ofs[k];
-      C.n[k] = m.cls_n[k];
-      C.block0[k] = blocks;
-      const int per_block = k == 0 ? 256 : 256 / kClsMax[k];
-      blocks += (C.n[k] + per_block - 1) / per_block;
-    }
-    C.block0[5] = blocks;
-    const int cap = m.max_large;
-    const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
-    const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
-    const size_t large_lds = (size_t)cap * 36 + (size_t)scap * 4;
-    if (c->tune.fork >= 2 && blocks && m.cls_n[kClsLarge]) {
-      // one launch for both kinds when two of its workgroups fit a CU's LDS
-      // (the small-net blocks then run W wavefronts each, as four 4-wavefront
-      // blocks would); with fork = 3 also at one workgroup per CU, the
-      // small-net blocks using as many group areas as the large tree's LDS
-      // holds; otherwise the second-stream branch below
-      const int Wmax = c->tune.large_threads / 64;
-      int W = std::min(8, Wmax);
-      size_t lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
-      bool one = 2 * (lds + 1024) <= (size_t)c->lds_max;
-      if (!one && c->tune.fork == 3) {
-        W = (int)std::min<size_t>((size_t)Wmax, std::max<size_t>(8, large_lds / sizeof(GroupLds)));
-        lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
-        one = lds + 1024 <= (size_t)c->lds_max;
-      }
-      if (one) {
-        SmallClasses Cm = C;
-        int mb = 0;
-        for (int k = 0; k < 5; ++k) {
-          Cm.block0[k] = mb;
-          const int per_block = k == 0 ? 64 * W : 64 * W / kClsMax[k];
-          mb += (Cm.n[k] + per_block - 1) / per_block;
-        }
-        Cm.block0[5] = mb;
-        A.list = s.d_list + m.cls_ofs[kClsLarge];
-        A.n_list = m.cls_n[kClsLarge];
-        hipLaunchKernelGGL(k_tree_mixed, dim3(A.n_list + mb), dim3(c->tune.large_threads), lds, c->stream, A, Cm,
-                           A.n_list, cap, scap, W);
-        goto trees_done;
-      }
-    }
-    {
-    // small and large nets are independent: with both present, the large-net
-    // workgroups run on a second stream (a parallel branch of the graph)
-    const bool fork = c->tune.fork && blocks && m.cls_n[kClsLarge] && !c->timing;
-    hipStream_t lst = c->stream;
-    if (fork) {
-      HIPC(c, hipEventRecord(c->ev_fork, c->stream));
-      HIPC(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-      lst = c->stream2;
-    }
-    if (blocks) hipLaunchKernelGGL(k_tree_small, dim3(blocks), dim3(256), 0, c->stream, A, C);
-    if (m.cls_n[kClsLarge]) {
-      A.list = s.d_list + m.cls_ofs[kClsLarge];
-      A.n_list = m.cls_n[kClsLarge];
-      hipLaunchKernelGGL(k_tree_large, dim3(A.n_list), dim3(c->tune.large_threads), large_lds, lst, A, cap, scap);
-    }
-    if (fork) {
-      HIPC(c, hipEventRecord(c->ev_join, c->stream2));
-      HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    }
-    }
-  trees_done:;
+    const int rc = launch_trees(c, s, m, A);
+    if (rc) return rc;
   }
   {
     KTimer t(c, GR_K_REFRESH);
