@@ -166,25 +166,35 @@ __device__ __forceinline__ float wterm(const float* __restrict__ pfx, const Geo&
   return Q(pfx, g, z, line, hi) - Q(pfx, g, z, line, lo - 1);
 }
 
-// Pair w of the batch -> batch position of the net that owns it.  chunk_net[c]
-// is the owner of pair 64*c; a net has >= 1 pair, so the owner of any pair of
-// chunk c is among the 64 positions starting there.  The wavefront of chunk c
-// loads that window of pair offsets once (coalesced) and each lane binary
-// searches it through lane shuffles.  Every lane of the wave must call this.
-__device__ __forceinline__ int pair_owner_wave(const int64_t* __restrict__ bpair,
-                                               const int32_t* __restrict__ chunk_net, int n, int64_t w) {
+// Pair w of the batch -> batch position of the net that owns it, with that
+// position's first pair and first pin.  chunk_net[c] is the owner of pair
+// 64*c; a net has >= 1 pair, so the owner of any pair of chunk c is among the
+// 64 positions starting there.  The wavefront of chunk c loads that window of
+// pair and pin offsets once (coalesced) and each lane binary searches it
+// through lane shuffles; the owner's offsets come from the window as well, so
+// no load waits on the search.  Every lane of the wave must call this.
+struct PairOwner {
+  int pos;
+  int64_t first_pair, first_pin;
+};
+__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
+  return (int64_t)(((uint64_t)(uint32_t)__shfl((int)(v >> 32), src, 64) << 32) | (uint32_t)__shfl((int)v, src, 64));
+}
+__device__ __forceinline__ PairOwner pair_owner_wave(const int64_t* __restrict__ bpair,
+                                                     const int64_t* __restrict__ pinofs,
+                                                     const int32_t* __restrict__ chunk_net, int n, int64_t w) {
   const int lane = threadIdx.x & 63;
   const int base = chunk_net[w >> 6];
-  const int64_t v = (base + lane < n) ? bpair[base + lane] : INT64_MAX;
+  const bool in = base + lane < n;
+  const int64_t v = in ? bpair[base + lane] : INT64_MAX;
+  const int64_t pv = in ? pinofs[base + lane] : 0;
   int lo = 0, hi = 64;  // v[lo] <= w < v[hi]
 #pragma unroll
   for (int it = 0; it < 6; ++it) {
     const int mid = (lo + hi) >> 1;
-    const int64_t vm = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(v >> 32), mid, 64) << 32) |
-                                 (uint32_t)__shfl((int)v, mid, 64));
-    if (vm <= w) lo = mid; else hi = mid;
+    if (shfl_i64(v, mid) <= w) lo = mid; else hi = mid;
   }
-  return base + lo;
+  return PairOwner{base + lo, shfl_i64(v, lo), shfl_i64(pv, lo)};
 }
 
 // Lexicographic minimum key (ord_key(cost) << 32 | p) over every pattern of

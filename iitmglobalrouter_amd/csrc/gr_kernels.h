@@ -177,11 +177,11 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
   const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
-  const int pos = pair_owner_wave(bpair, chunk_net, n, w);
+  const PairOwner own = pair_owner_wave(bpair, pinofs, chunk_net, n, w);
   if (w0 >= P) return;
   int j, k;
-  pair_decode((uint32_t)(w - bpair[pos]), j, k);
-  const int64_t pb = pinofs[pos];
+  pair_decode((uint32_t)(w - own.first_pair), j, k);
+  const int64_t pb = own.first_pin;
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);

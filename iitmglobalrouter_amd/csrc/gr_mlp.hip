@@ -365,12 +365,12 @@ __global__ __launch_bounds__(256) void k_lpat_topk(Geo g, const float* __restric
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp
   const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t w = w0 < P ? w0 : P - 1;
-  const int pos = pair_owner_wave(bpair, chunk_net, n, w);
+  const PairOwner own = pair_owner_wave(bpair, pinofs, chunk_net, n, w);
   bool fb = false;
   if (w0 < P) {
     int j, k;
-    pair_decode((uint32_t)(w - bpair[pos]), j, k);
-    const int64_t pb = pinofs[pos];
+    pair_decode((uint32_t)(w - own.first_pair), j, k);
+    const int64_t pb = own.first_pin;
     int z1, x1, y1, z2, x2, y2;
     dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
     dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
@@ -622,10 +622,10 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     for (int i = tid; i < kLut; i += NT) slut[i] = f2bf(A.loglut[i]);
     if (tid < TT) {  // waves 0 and 1 whole: the owner search is wave-collective
       const int64_t w = t0 + (tid < nrow ? tid : nrow - 1);
-      const int pos = pair_owner_wave(A.bpair, A.chunk_net, A.n, w);
+      const PairOwner own = pair_owner_wave(A.bpair, A.pinofs, A.chunk_net, A.n, w);
       int j, k;
-      pair_decode((uint32_t)(w - A.bpair[pos]), j, k);
-      const int64_t pb = A.pinofs[pos];
+      pair_decode((uint32_t)(w - own.first_pair), j, k);
+      const int64_t pb = own.first_pin;
       int z1, x1, y1, z2, x2, y2;
       dec(g, (uint32_t)A.pins[pb + j], z1, x1, y1);
       dec(g, (uint32_t)A.pins[pb + k], z2, x2, y2);
