@@ -402,7 +402,11 @@ def main():
     deg, P, F = footprint(X, Y, Z, pfx, pins)
     G = int(r.demand().sum())  # committed cells over the pass = total demand bumps
     n_b = len(ofst) - 1
-    # algorithmic bytes per pass of each kernel (per-unit figures in DESIGN.md)
+    # algorithmic bytes per pass of each kernel, summed per net: costing reads
+    # the pins (4 B each) and the distinct cost cells its pairs depend on
+    # (4 B each) and writes a cost and a pattern per pair (6 B); the trees read
+    # every pair cost, the pins, the edge patterns and slots, and stamp 4 B per
+    # committed cell
     kbytes = {
         0: float((4 * deg + 4 * F + 6 * P).sum()),                        # pins + cost footprint + (cost, pattern) out
         1: float((4 * P + 4 * deg + 2 * (deg - 1) + 12 * deg + 4).sum() + 4 * G),  # costs, pins, patterns, slots, stamps
