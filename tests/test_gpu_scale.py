@@ -46,6 +46,47 @@ def test_huge_nets(gr, orc, gpu):
     assert np.array_equal(r.demand(), o.demand())
 
 
+def test_mixed_batches_with_huge_nets(gr, orc, gpu):
+    """Batches holding large and small nets together run as one mixed tree
+    launch: with a net whose tree needs most of the LDS (one workgroup per CU,
+    up to 15 small-net wavefronts per workgroup) and with moderate nets (two
+    workgroups per CU).  Bit-exact with the oracle, and identical to the
+    second-stream form (fork = 1)."""
+    X, Y, Z = 256, 256, 10
+    cap = gr.gen_grid(2, 11, X, Y, Z)
+    rng = np.random.default_rng(11)
+    nets = []
+    for npins in (2500, 900, 90, 40, 7, 2, 333, 170, 60, 12, 3, 170, 90, 33, 9, 2):
+        s = set()
+        while len(s) < npins:
+            s.add(_enc(X, Y, int(rng.integers(0, 2)), int(rng.integers(0, X)), int(rng.integers(0, Y))))
+        nets.append(sorted(s))
+    pfx = np.cumsum([0] + [len(n) for n in nets]).astype(np.int64)
+    pins = np.concatenate([np.array(n, np.int32) for n in nets])
+    batches = [list(range(0, 6)), list(range(6, 11)), list(range(11, 16))]
+    outs = {}
+    for fork in (3, 1):
+        r = gr.Router(X, Y, Z, cap)
+        r.set_tuning(fork=fork)
+        r.set_nets(pfx, pins)
+        o = orc.Oracle(X, Y, Z, cap) if fork == 3 else None
+        if o is not None:
+            o.set_nets(pfx, pins)
+        res = []
+        for b, batch in enumerate(batches):
+            print(f"fork {fork} batch {b} pins {[len(nets[i]) for i in batch]}", flush=True)
+            e1, l1, c1 = r.route_batch(batch)
+            res.append((e1, l1, c1.view(np.int32), r.demand().copy()))
+            if o is not None:
+                e2, l2, c2 = o.route_batch(batch)
+                assert np.array_equal(e1, e2) and np.array_equal(l1, l2) and _same_f32(c1, c2), b
+                assert np.array_equal(r.demand(), o.demand()), b
+        outs[fork] = res
+    for a, b in zip(outs[3], outs[1]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("kind,X,nets", [(2, 800, 150000), (3, 2000, 1500000), (5, 2000, 1500000)])
 def test_full_workload_sampled_batches(gr, orc, gpu, kind, X, nets):
     Y, Z = X, 10
