@@ -250,7 +250,7 @@ def stress_eval(gr, args, tuning):
     r.set_nets(pfx, pins)
     r.set_schedule(ofst, ids)
     if tuning:
-        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave"]
+        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave", "lpat_block"]
         r.set_tuning(**{k: tuning[k] for k in keys})
     print(f"# stress setup {time.perf_counter() - t:.2f}s nets {n_nets} batches {nb}", file=sys.stderr, flush=True)
     r.reset_demand()
@@ -333,7 +333,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
     ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
     ap.add_argument("--autotune", type=int, default=1, help="pick schedule parameters by timing (0: defaults)")
-    ap.add_argument("--tuning", default="", help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave")
+    ap.add_argument("--tuning", default="",
+                    help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave[,lpat_block]")
     ap.add_argument("--io", type=int, default=1, help="also measure evaluator, writer and reader (0: skip)")
     ap.add_argument("--mst", type=int, default=1, help="also time true-MST trees for route quality (0: skip)")
     ap.add_argument("--stress", type=int, default=1,
@@ -363,7 +364,7 @@ def main():
     n_nets = len(pfx) - 1
     tuning = None
     if args.tuning:
-        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave"]
+        keys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave", "lpat_block"]
         r.set_tuning(**dict(zip(keys, (int(v) for v in args.tuning.split(",")))))
         tuning = r.tuning().as_dict()
     elif args.autotune:

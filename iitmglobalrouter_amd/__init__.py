@@ -44,7 +44,7 @@ class RouterError(RuntimeError):
 class Tuning(ctypes.Structure):
     """Schedule parameters (speed only, never results)."""
     _fields_ = [("large_threads", ctypes.c_int32), ("group_max", ctypes.c_int32), ("fork", ctypes.c_int32),
-                ("refresh_blocks", ctypes.c_int32), ("lpat_wave", ctypes.c_int32)]
+                ("refresh_blocks", ctypes.c_int32), ("lpat_wave", ctypes.c_int32), ("lpat_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // L-pattern costing
 // ---------------------------------------------------------------------------
 
-// One lane per pin pair of the batch.
+// One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
                                                    const float* __restrict__ vterm,
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
                                                    float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
                                                    int32_t* __restrict__ ctr) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
-  const int64_t w0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
   const PairOwner own = pair_owner_wave(bpair, pinofs, chunk_net, n, w);
   if (w0 >= P) return;

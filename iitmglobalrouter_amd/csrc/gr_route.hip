@@ -292,7 +292,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     KTimer t(c, GR_K_LPAT);
     LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
     if (fn) {
-      hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + 255) / 256)), dim3(256), 0, c->stream, c->g, c->d_pfx,
+      hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + c->tune.lpat_block - 1) / c->tune.lpat_block)), dim3(c->tune.lpat_block), 0, c->stream, c->g, c->d_pfx,
                          c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
                          c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     } else {
@@ -663,7 +663,8 @@ int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
   if (!c || !t) return fail(c, GR_EINVAL, "null argument");
   if ((t->large_threads != 256 && t->large_threads != 512 && t->large_threads != 1024) ||
       (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || t->fork < 0 || t->fork > 3 ||
-      (t->lpat_wave != 0 && t->lpat_wave != 1))
+      (t->lpat_wave != 0 && t->lpat_wave != 1) ||
+      (t->lpat_block != 64 && t->lpat_block != 128 && t->lpat_block != 256))
     return fail(c, GR_EINVAL, "tuning out of range");
   (void)hipSetDevice(c->dev);
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -706,31 +707,39 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
   double win_ms = 1e30;
   const int nb = (int)c->sched.meta.size();
   int rc = 0;
+  // one untimed pass captures the graph for the candidate, then reps timed passes
+  auto time_one = [&](const gr_tuning& t) -> int {
+    int r0 = gr_set_tuning(c, &t);
+    if (r0 || (r0 = gr_reset_demand(c)) || (r0 = gr_route_schedule(c, 0, nb))) return r0;
+    float ms = 0.f;
+    double tot = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      if ((r0 = gr_reset_demand(c))) return r0;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(c, GR_ENODEV, "event");
+      if ((r0 = gr_route_schedule(c, 0, nb))) return r0;
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        return fail(c, GR_ENODEV, "event timing failed");
+      tot += ms;
+    }
+    if (tot / reps < win_ms) { win_ms = tot / reps; win = t; }
+    return 0;
+  };
   for (int gm : {64, 32})
     for (int lt : {1024, 512, 256})
       for (int fk : {3, 1, 0})
         for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
             if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z
-            gr_tuning t = {lt, gm, fk, rb, lw};
-            if ((rc = gr_set_tuning(c, &t))) goto done;
-            // one untimed pass captures the graph for this candidate
-            if ((rc = gr_reset_demand(c)) || (rc = gr_route_schedule(c, 0, nb))) goto done;
-            float ms = 0.f;
-            double tot = 0.0;
-            for (int r = 0; r < reps; ++r) {
-              if ((rc = gr_reset_demand(c))) goto done;
-              if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, GR_ENODEV, "event"); goto done; }
-              if ((rc = gr_route_schedule(c, 0, nb))) goto done;
-              if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                rc = fail(c, GR_ENODEV, "event timing failed");
-                goto done;
-              }
-              tot += ms;
-            }
-            if (tot / reps < win_ms) { win_ms = tot / reps; win = t; }
+            if ((rc = time_one(gr_tuning{lt, gm, fk, rb, lw, 256}))) goto done;
           }
+  // then the lane kernel's workgroup size, around the winner
+  if (!win.lpat_wave)
+    for (int lb : {128, 64}) {
+      gr_tuning t = win;
+      t.lpat_block = lb;
+      if ((rc = time_one(t))) goto done;
+    }
 done:
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
