@@ -81,7 +81,7 @@ struct gr_ctx {
   void* d_eval_part = nullptr;
   void* d_eval_out = nullptr;
   size_t lds_max = 65536;
-  gr_tuning tune = {1024, 64, 3, 2048, 0, 256};
+  gr_tuning tune = {1024, 64, 3, 2048, 0, 0};
   int32_t n_nets = 0;
   std::vector<int64_t> h_pin_pfx;
   int64_t* d_pin_pfx = nullptr;

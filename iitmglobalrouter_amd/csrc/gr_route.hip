@@ -198,6 +198,17 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
 typedef void (*LpatFn)(Geo, const float*, const float*, const int32_t*, const int64_t*, const int64_t*,
                        const int32_t*, int, int64_t, float*, int16_t*, int32_t*);
 
+// Workgroup size of the lane costing kernel for a batch of P pairs: the
+// tuned size, or with 0 the largest of 256, 128, 64 that still gives every
+// CU several workgroups (at least 2048 in all, 8 per CU).
+constexpr int64_t kLpatMinBlocks = 2048;
+int lpat_block_for(int tuned, int64_t P) {
+  if (tuned) return tuned;
+  for (int b : {256, 128})
+    if ((P + b - 1) / b >= kLpatMinBlocks) return b;
+  return 64;
+}
+
 LpatFn lane_kernel(int Z) {
   switch (Z) {
     case 3: return k_lpat_lane<3>;
@@ -292,7 +303,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     KTimer t(c, GR_K_LPAT);
     LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
     if (fn) {
-      hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + c->tune.lpat_block - 1) / c->tune.lpat_block)), dim3(c->tune.lpat_block), 0, c->stream, c->g, c->d_pfx,
+      const int lb = lpat_block_for(c->tune.lpat_block, m.P);
+      hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
                          c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
                          c->d_pair_cost, c->d_pair_pat, c->d_ctr);
     } else {
@@ -664,7 +676,7 @@ int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
   if ((t->large_threads != 256 && t->large_threads != 512 && t->large_threads != 1024) ||
       (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || t->fork < 0 || t->fork > 3 ||
       (t->lpat_wave != 0 && t->lpat_wave != 1) ||
-      (t->lpat_block != 64 && t->lpat_block != 128 && t->lpat_block != 256))
+      (t->lpat_block != 0 && t->lpat_block != 64 && t->lpat_block != 128 && t->lpat_block != 256))
     return fail(c, GR_EINVAL, "tuning out of range");
   (void)hipSetDevice(c->dev);
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -731,11 +743,11 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
         for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
             if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z
-            if ((rc = time_one(gr_tuning{lt, gm, fk, rb, lw, 256}))) goto done;
+            if ((rc = time_one(gr_tuning{lt, gm, fk, rb, lw, 0}))) goto done;
           }
   // then the lane kernel's workgroup size, around the winner
   if (!win.lpat_wave)
-    for (int lb : {128, 64}) {
+    for (int lb : {256, 128, 64}) {
       gr_tuning t = win;
       t.lpat_block = lb;
       if ((rc = time_one(t))) goto done;

@@ -191,7 +191,7 @@ int gr_lut_size(void);
 
 /*
  * Schedule parameters.  They change speed, never results (every choice is
- * bit-identical).  Defaults: {1024, 64, 3, 2048, 0, 256}.
+ * bit-identical).  Defaults: {1024, 64, 3, 2048, 0, 0}.
  *   large_threads   workgroup size for nets above group_max pins (256, 512 or 1024)
  *   group_max       largest pin count handled by a wavefront group of <= 64 lanes (32 or 64)
  *   fork            how a batch with small and large nets runs its trees:
@@ -201,7 +201,9 @@ int gr_lut_size(void);
  *                   for both whenever the large tree fits in LDS
  *   refresh_blocks  persistent grid of the cost refresh (>= 1)
  *   lpat_wave       1: one wavefront per pin pair instead of one lane per pair
- *   lpat_block      workgroup size of the lane-per-pair costing (64, 128 or 256)
+ *   lpat_block      workgroup size of the lane-per-pair costing (64, 128 or 256), or
+ *                   0: per batch, the largest of these that still gives >= 2048
+ *                   workgroups (64 below that)
  */
 typedef struct {
   int32_t large_threads, group_max, fork, refresh_blocks, lpat_wave, lpat_block;

@@ -121,7 +121,7 @@ def test_tunings_are_bit_identical(gr, gpu):
               dict(fork=2, large_threads=512, group_max=32), dict(fork=2, large_threads=256, group_max=64),
               dict(fork=3, large_threads=1024, group_max=32), dict(fork=3, large_threads=256, group_max=32),
               dict(fork=1, large_threads=1024, group_max=64), dict(fork=3, lpat_block=64),
-              dict(fork=3, lpat_block=128, group_max=32)):
+              dict(fork=3, lpat_block=128, group_max=32), dict(fork=1, lpat_block=256)):
         r.set_tuning(**t)
         r.reset_demand()
         r.route_schedule()
