@@ -289,15 +289,16 @@ def stress_eval(gr, args, tuning):
     r.proposal_stats()
     r.sync()
     ta = time.perf_counter()
-    rounds = 0
+    rounds = skipped = 0
     for b0 in range(0, nb, 50):
         r.route_schedule(b0, min(50, nb - b0))
         if b0 + 50 < nb:  # fine-tune on this chunk's fallback-judged samples
             try:
                 m.train(epochs=1, batch=1024, lr=0.02, seed=1000 + b0)
                 rounds += 1
-            except gr.RouterError:
-                pass  # fewer labelled samples than one minibatch
+            except gr.RouterError as e:  # fewer labelled samples than one minibatch
+                skipped += 1
+                print(f"# stress fine-tune after batch {b0 + 50} skipped: {e}", file=sys.stderr, flush=True)
             m.clear_samples()
     r.sync()
     learned_ms = (time.perf_counter() - ta) * 1e3
@@ -317,6 +318,7 @@ def stress_eval(gr, args, tuning):
             "vs_plain": round(plain_ms / learned_ms, 4), "route_cost": learned_cost,
             "route_cost_gap": round((learned_cost["total"] - plain_cost["total"]) / plain_cost["total"], 6),
             "fallback_rate": round(fb / max(pairs, 1), 4), "finetune_rounds": rounds,
+            "finetune_skipped": skipped, "finetune_points": len(range(50, nb, 50)),
             "pretrain": {"seeds": args.stress_seeds, "s": round(pretrain_s, 2)}},
     }
 

@@ -35,23 +35,5 @@ def build(verbose=False):
     return LIB
 
 
-PROBE_SRC = os.path.join(HERE, "probes", "launch_overheads.hip")
-PROBE = os.path.join(HERE, "probes", "launch_overheads")
-
-
-def build_probe(verbose=False):
-    """The launch-overhead probe (a stand-alone program, not part of the library)."""
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", "-o", PROBE + ".tmp", PROBE_SRC]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
-
-
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -295,6 +295,9 @@ __device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int 
 // via cells, inclusive wires.  With the quirk, the k threads that marked the
 // first via stack also mark the k cells just before the first wire's low end,
 // by enc arithmetic (:213-216: the first wire call is not guarded by tid >= 0).
+// A quirk cell whose enc would be negative (possible only when X*Y < Z) lies
+// outside the grid: the reference writes before its array there; it is
+// skipped here.
 __device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
   const Geo& g = A.g;
   if (p < g.Z2 || p >= g.Z * g.Z2) return;  // no valid pattern (cannot happen for Z >= 3)
@@ -312,12 +315,18 @@ __device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p
   if (z & 1) {
     const int lo = min(y, y2), hi = max(y, y2);
     mark_wire(A, z, x, lo, hi);
-    if (A.quirk) for (int t = 1; t <= k; ++t) mark_any(A, enc(g, z, x, lo) - t);
+    if (A.quirk) {
+      const uint32_t e0 = enc(g, z, x, lo);
+      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, e0 - t);
+    }
     y = y2;
   } else {
     const int lo = min(x, x2), hi = max(x, x2);
     mark_wire(A, z, y, lo, hi);
-    if (A.quirk) for (int t = 1; t <= k; ++t) mark_any(A, enc(g, z, lo, y) - t);
+    if (A.quirk) {
+      const uint32_t e0 = enc(g, z, lo, y);
+      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, e0 - t);
+    }
     x = x2;
   }
   const int bp = (z & 1) ? 2 * b : 2 * b + 1;

@@ -102,6 +102,9 @@ int refresh_all(gr_ctx* c) {
 
 // Per-position descriptors of a schedule, built on the host and uploaded once.
 int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofst, const int32_t* net_ids) {
+  // The captured whole-pass graph holds the resident schedule's batch
+  // boundaries and device pointers: rebuilding that schedule invalidates it.
+  if (&s == &c->sched) drop_graph(c);
   const int32_t npos = batch_ofst[n_batches];
   std::vector<int64_t> slot(npos), pair(npos), pinofs(npos);
   std::vector<int32_t> degv(npos);
@@ -177,6 +180,9 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     s.cap_chunk = chunk.size();
   }
   if (maxP > c->pair_cap) {
+    // the pair buffers are the context's, shared by every schedule: the
+    // captured pass must not keep their old addresses
+    drop_graph(c);
     if ((rc = dalloc(c, &c->d_pair_cost, maxP)) || (rc = dalloc(c, &c->d_pair_pat, maxP)) ||
         (rc = dalloc(c, &c->d_pair_fb, maxP)))
       return rc;

@@ -260,14 +260,14 @@ struct Ctx {
       if (zw & 1) {
         int lo = std::min(y, y2), hi = std::max(y, y2);
         for (int t = lo; t <= hi; ++t) out.push_back(enc(zw, x, t));
-        if (first && commit_mode == 0)
-          for (int t = 1; t <= k; ++t) out.push_back(enc(zw, x, lo) - t);
+        if (first && commit_mode == 0)  // quirk cells below enc 0 (X*Y < Z only) are outside the grid: skipped
+          for (int t = 1; t <= k && enc(zw, x, lo) - t >= 0; ++t) out.push_back(enc(zw, x, lo) - t);
         y = y2;
       } else {
         int lo = std::min(x, x2), hi = std::max(x, x2);
         for (int t = lo; t <= hi; ++t) out.push_back(enc(zw, t, y));
         if (first && commit_mode == 0)
-          for (int t = 1; t <= k; ++t) out.push_back(enc(zw, lo, y) - t);
+          for (int t = 1; t <= k && enc(zw, lo, y) - t >= 0; ++t) out.push_back(enc(zw, lo, y) - t);
         x = x2;
       }
     };

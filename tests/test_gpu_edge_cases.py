@@ -133,3 +133,19 @@ def test_empty_batches(gr, orc, gpu):
     r2.set_nets(pfx, pins)
     e_ref = np.concatenate([r2.route_batch(ids[ofst[b]:ofst[b + 1]])[0] for b in range(len(ofst) - 1)])
     assert np.array_equal(edges, e_ref)
+
+
+@pytest.mark.parametrize("X,Y,Z", [(1, 1, 6), (1, 2, 12), (2, 1, 9)])
+def test_quirk_cells_below_the_grid(gr, orc, gpu, X, Y, Z):
+    """Grids with X*Y < Z: the quirk's extra cells before the first wire can
+    fall below enc 0; both sides skip them (no write outside the grid)."""
+    nets = []
+    for x in range(X):
+        for y in range(Y):
+            nets.append([(Z - 1, x, y), (0, x, y)])
+            nets.append([(Z - 1, x, y), (Z // 2, x, y), (1, x, y)])
+    if X * Y > 1:
+        nets.append([(Z - 1, 0, 0), (0, X - 1, Y - 1)])
+    pfx, pins = _encode(X, Y, Z, nets)
+    cap = np.full(X * Y * Z, 2, np.int32)
+    _run(gr, orc, X, Y, Z, cap, pfx, pins, 0, 0, f"tiny {X}x{Y}x{Z}")

@@ -53,10 +53,13 @@ def test_initial_state(gr, orc, gpu):
         assert not r.demand().any()
 
 
-def test_workload1_full(gr, orc, gpu):
-    """128x128x6, 5,000 nets, seed 0: every batch bit-exact."""
-    r, o, ofst, ids = _setup(gr, orc, 1, 0, 128, 128, 6, 5000)
-    _compare_batches(r, o, ofst, ids, label="w1")
+@pytest.mark.parametrize("cost_model", [0, 2])
+def test_workload1_full(gr, orc, gpu, cost_model):
+    """128x128x6, 5,000 nets, seed 0: every batch bit-exact, with the BIT cost
+    exp(-r/2) and with the literal exp(d + 1 - c) = exp(1 - r) the workload
+    names."""
+    r, o, ofst, ids = _setup(gr, orc, 1, 0, 128, 128, 6, 5000, cost_model)
+    _compare_batches(r, o, ofst, ids, label=f"w1m{cost_model}")
 
 
 @pytest.mark.parametrize("cost_model,commit_mode", [(1, 0), (2, 0), (0, 1)])

@@ -270,3 +270,21 @@ def test_route_batch_commit_consistency(orc, gr):
         assert set(np.nonzero(delta)[0].tolist()) == cells
         assert delta.max() <= 1 and delta.min() >= 0
         prev = cur
+
+
+def test_quirk_cells_below_the_grid_are_skipped(orc):
+    """With X*Y < Z the quirk's cells before the first wire can fall below
+    enc 0 (the reference writes before its flag array there); they are
+    skipped.  1x1x6 grid, src layer 5 -> dst layer 0, pattern a = 1: interior
+    vias 2..4, the wire cell 1, then quirk cells 1-1 = 0 (kept), -1, -2
+    (skipped)."""
+    X, Y, Z = 1, 1, 6
+    o = orc.Oracle(X, Y, Z, np.full(X * Y * Z, 4, np.int32), 0, 0)
+    cells = o.edge_cells(5, 0, 1 * 3 + 0)
+    assert sorted(cells.tolist()) == [0, 1, 2, 3, 4]
+    o_fixed = orc.Oracle(X, Y, Z, np.full(X * Y * Z, 4, np.int32), 0, 1)
+    assert sorted(o_fixed.edge_cells(5, 0, 3).tolist()) == [1, 2, 3, 4]
+    # routed: every cell of the stack gets demand once, nothing outside the grid
+    o.set_nets(np.array([0, 2], np.int64), np.array([0, 5], np.int32))
+    o.route_batch(np.array([0], np.int32))
+    assert o.demand().tolist() == [1, 1, 1, 1, 1, 0]
