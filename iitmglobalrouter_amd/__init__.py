@@ -75,6 +75,8 @@ _SIGS = {
     "gr_destroy": (None, [_P]),
     "gr_set_nets": (ctypes.c_int, [_P, _I32, _P, _P]),
     "gr_route_batch": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
+    "gr_route_batch_static": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
+    "gr_static_scratch_bytes": (_I64, []),
     "gr_set_schedule": (ctypes.c_int, [_P, _I32, _P, _P]),
     "gr_route_schedule": (ctypes.c_int, [_P, _I32, _I32]),
     "gr_sync": (ctypes.c_int, [_P]),
@@ -380,6 +382,17 @@ class Router:
         lpat = np.empty(slots, dtype=np.int32)
         cost = np.empty(len(ids), dtype=np.float32)
         self._c(self.lib.gr_route_batch(self.h, _ptr(ids), len(ids), _ptr(edges), _ptr(lpat), _ptr(cost)))
+        return edges, lpat, cost
+
+    def route_batch_static(self, net_ids):
+        """route_batch without any device allocation (gr_route_batch_static):
+        the batch's device buffers live in the library's static per-GPU array."""
+        ids = np.ascontiguousarray(net_ids, dtype=np.int32)
+        slots = int(self.deg[ids].sum()) if len(ids) else 0
+        edges = np.empty(slots, dtype=np.int64)
+        lpat = np.empty(slots, dtype=np.int32)
+        cost = np.empty(len(ids), dtype=np.float32)
+        self._c(self.lib.gr_route_batch_static(self.h, _ptr(ids), len(ids), _ptr(edges), _ptr(lpat), _ptr(cost)))
         return edges, lpat, cost
 
     def set_schedule(self, batch_ofst, net_ids):

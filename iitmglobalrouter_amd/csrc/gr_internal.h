@@ -54,6 +54,18 @@ struct Sched {
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;
+  // ext: every device array above, and the pair buffers below, point into the
+  // allocation-free entry point's static array (never freed, never grown)
+  bool ext = false;
+  float* x_pair_cost = nullptr;
+  int16_t* x_pair_pat = nullptr;
+};
+
+// Per-position descriptors of a schedule, computed on the host before upload.
+struct HostSched {
+  std::vector<int64_t> slot, pair, pinofs;
+  std::vector<int32_t> degv, list, chunk;
+  int64_t slots = 0, maxP = 0;
 };
 
 struct EvPair { hipEvent_t a, b; int kind; };
@@ -91,6 +103,10 @@ struct gr_ctx {
   int16_t* d_pair_pat = nullptr;
   uint8_t* d_pair_fb = nullptr;  // per pair of the batch: 1 when the proposal fell back to every pattern
   int64_t pair_cap = 0, last_P = 0;
+  const float* last_pair_cost = nullptr;  // where the last batch's per-pair results are
+  const int16_t* last_pair_pat = nullptr;
+  void* scratch = nullptr;  // this device's static array of the allocation-free entry point
+  grh::HostSched hst;       // host descriptors of that entry point, reused across calls
   // learned proposal: when set, costing evaluates only the model's top-k patterns
   gr_mlp* prop = nullptr;
   int prop_k = 8;

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,33 @@ void build_lut(int model, float* lut) {
 
 namespace grh {
 std::string g_err;
+
+// Allocation-free routing (gr_route_batch_static): one static device array per
+// GPU holds every device buffer a batch needs beyond the context's grid state
+// and nets.  Host code never uses the array's name as an address: the device
+// address comes from hipGetSymbolAddress, once per device, kept below.
+constexpr size_t kScratchBytes = (size_t)GR_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's array; the device must be current.
+int scratch_address(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, GR_ERANGE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, GR_ENODEV, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
 
 void drop_graph(gr_ctx* c) {
   Sched& s = c->sched;
@@ -100,15 +128,15 @@ int refresh_all(gr_ctx* c) {
   return 0;
 }
 
-// Per-position descriptors of a schedule, built on the host and uploaded once.
-int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofst, const int32_t* net_ids) {
-  // The captured whole-pass graph holds the resident schedule's batch
-  // boundaries and device pointers: rebuilding that schedule invalidates it.
-  if (&s == &c->sched) drop_graph(c);
+int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32_t* batch_ofst,
+               const int32_t* net_ids) {
   const int32_t npos = batch_ofst[n_batches];
-  std::vector<int64_t> slot(npos), pair(npos), pinofs(npos);
-  std::vector<int32_t> degv(npos);
-  std::vector<int32_t> list(npos), chunk;
+  h.slot.assign(npos, 0);
+  h.pair.assign(npos, 0);
+  h.pinofs.assign(npos, 0);
+  h.degv.assign(npos, 0);
+  h.list.assign(npos, 0);
+  h.chunk.clear();
   s.meta.assign(n_batches, BatchMeta{});
   int64_t slots = 0, maxP = 0;
   int32_t lo = 0;
@@ -127,10 +155,10 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
       if (d > kLargeDeg)
         return fail(c, GR_ERANGE, "net has more pins than the device tree supports (" + std::to_string(kLargeDeg) + ")");
-      slot[m.begin + i] = slots;
-      pair[m.begin + i] = P;
-      pinofs[m.begin + i] = c->h_pin_pfx[id];
-      degv[m.begin + i] = (int32_t)d;
+      h.slot[m.begin + i] = slots;
+      h.pair[m.begin + i] = P;
+      h.pinofs[m.begin + i] = c->h_pin_pfx[id];
+      h.degv[m.begin + i] = (int32_t)d;
       slots += d;
       P += d * (d - 1) / 2;
     }
@@ -142,7 +170,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
         const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
         const int k = net_class(d, c->tune.group_max);
         if (k != cls) continue;
-        list[lo++] = i;
+        h.list[lo++] = i;
         if (k == kClsLarge) {
           m.max_large = std::max<int>(m.max_large, (int)d);
           m.max_large_P = std::max<int64_t>(m.max_large_P, d * (d - 1) / 2);
@@ -151,11 +179,11 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       m.cls_n[cls] = lo - m.cls_ofs[cls];
     }
     // owner of pair 64*c for every 64-pair chunk of the batch
-    m.chunk_ofs = (int32_t)chunk.size();
+    m.chunk_ofs = (int32_t)h.chunk.size();
     int owner = 0;
     for (int64_t w = 0; w < P; w += 64) {
-      while (owner + 1 < m.n && pair[m.begin + owner + 1] <= w) ++owner;
-      chunk.push_back(owner);
+      while (owner + 1 < m.n && h.pair[m.begin + owner + 1] <= w) ++owner;
+      h.chunk.push_back(owner);
     }
     m.P = P;
     maxP = std::max(maxP, P);
@@ -163,8 +191,40 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   s.n_pos = npos;
   s.slots = slots;
   s.max_P = maxP;
-  int rc;
-  if ((size_t)npos > s.cap_pos) {
+  h.slots = slots;
+  h.maxP = maxP;
+  return 0;
+}
+
+// Copies the host descriptors into the schedule's device arrays.
+int upload_sched(gr_ctx* c, Sched& s, const HostSched& h) {
+  const size_t npos = h.slot.size();
+  if (npos) {
+    HIPC(c, hipMemcpyAsync(s.d_pinofs, h.pinofs.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_deg, h.degv.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_slot, h.slot.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_pair, h.pair.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(s.d_list, h.list.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
+  }
+  if (!h.chunk.empty())
+    HIPC(c, hipMemcpyAsync(s.d_chunk, h.chunk.data(), sizeof(int32_t) * h.chunk.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));  // the host vectors may change after this
+  return 0;
+}
+
+// Per-position descriptors of a schedule, built on the host and uploaded once
+// into device arrays the schedule owns (grown as needed).
+int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofst, const int32_t* net_ids) {
+  // The captured whole-pass graph holds the resident schedule's batch
+  // boundaries and device pointers: rebuilding that schedule invalidates it.
+  if (&s == &c->sched) drop_graph(c);
+  HostSched h;
+  int rc = host_sched(c, s, h, n_batches, batch_ofst, net_ids);
+  if (rc) return rc;
+  const size_t npos = h.slot.size();
+  const int64_t slots = h.slots, maxP = h.maxP;
+  if (npos > s.cap_pos) {
     if ((rc = dalloc(c, &s.d_pinofs, npos)) || (rc = dalloc(c, &s.d_deg, npos)) ||
         (rc = dalloc(c, &s.d_slot, npos)) || (rc = dalloc(c, &s.d_pair, npos)) ||
         (rc = dalloc(c, &s.d_list, npos)) || (rc = dalloc(c, &s.d_cost, npos)))
@@ -175,9 +235,9 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
     if ((rc = dalloc(c, &s.d_edges, slots)) || (rc = dalloc(c, &s.d_lpat, slots))) return rc;
     s.cap_slots = slots;
   }
-  if (chunk.size() > s.cap_chunk) {
-    if ((rc = dalloc(c, &s.d_chunk, chunk.size()))) return rc;
-    s.cap_chunk = chunk.size();
+  if (h.chunk.size() > s.cap_chunk) {
+    if ((rc = dalloc(c, &s.d_chunk, h.chunk.size()))) return rc;
+    s.cap_chunk = h.chunk.size();
   }
   if (maxP > c->pair_cap) {
     // the pair buffers are the context's, shared by every schedule: the
@@ -188,17 +248,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       return rc;
     c->pair_cap = maxP;
   }
-  if (npos) {
-    HIPC(c, hipMemcpyAsync(s.d_pinofs, pinofs.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMemcpyAsync(s.d_deg, degv.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMemcpyAsync(s.d_slot, slot.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMemcpyAsync(s.d_pair, pair.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMemcpyAsync(s.d_list, list.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
-  }
-  if (!chunk.empty())
-    HIPC(c, hipMemcpyAsync(s.d_chunk, chunk.data(), sizeof(int32_t) * chunk.size(), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
-  return 0;
+  return upload_sched(c, s, h);
 }
 
 typedef void (*LpatFn)(Geo, const float*, const float*, const int32_t*, const int64_t*, const int64_t*,
@@ -301,6 +351,8 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
 int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   const BatchMeta& m = s.meta[b];
   if (m.n == 0) return 0;
+  float* pcost = s.ext ? s.x_pair_cost : c->d_pair_cost;
+  int16_t* ppat = s.ext ? s.x_pair_pat : c->d_pair_pat;
   if (m.P > 0 && c->prop) {
     KTimer t(c, GR_K_LPAT);
     const int rc = mlp_enqueue_costing(c, s, m);
@@ -312,11 +364,11 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       const int lb = lpat_block_for(c->tune.lpat_block, m.P);
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
                          c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
-                         c->d_pair_cost, c->d_pair_pat, c->d_ctr);
+                         pcost, ppat, c->d_ctr);
     } else {
       hipLaunchKernelGGL(k_lpat_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
-                         c->d_pair_cost, c->d_pair_pat, c->d_ctr);
+                         pcost, ppat, c->d_ctr);
     }
   }
   if (m.P > 0 && c->collect) {
@@ -331,8 +383,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.pair = s.d_pair + m.begin;
   A.cost_base = m.begin;
   A.pins = c->d_pins;
-  A.pair_cost = c->d_pair_cost;
-  A.pair_pat = c->d_pair_pat;
+  A.pair_cost = pcost;
+  A.pair_pat = ppat;
   A.out_edges = s.d_edges;
   A.out_lpat = s.d_lpat;
   A.out_cost = s.d_cost;
@@ -354,6 +406,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   }
   HIPC(c, hipGetLastError());
   c->last_P = m.P;
+  c->last_pair_cost = pcost;
+  c->last_pair_pat = ppat;
   return 0;
 }
 
@@ -407,6 +461,10 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   gr_ctx* c = new gr_ctx();
   c->dev = d->device;
   if (hipSetDevice(c->dev) != hipSuccess) { delete c; return fail(nullptr, GR_ENODEV, "hipSetDevice failed"); }
+  {
+    const int rc0 = scratch_address(c->dev, &c->scratch);
+    if (rc0) { delete c; return rc0; }
+  }
   c->g.X = d->X; c->g.Y = d->Y; c->g.Z = d->Z; c->g.Z2 = (d->Z + 1) / 2;
   c->g.XY = (uint32_t)d->X * (uint32_t)d->Y;
   c->model = d->cost_model;
@@ -547,6 +605,69 @@ int gr_route_batch(gr_ctx* c, const int32_t* net_ids, int32_t n, int64_t* out_ed
   return 0;
 }
 
+int gr_route_batch_static(gr_ctx* c, const int32_t* net_ids, int32_t n, int64_t* out_edges, int32_t* out_lpat,
+                          float* out_net_cost) {
+  if (!c || n < 0 || (n && !net_ids)) return fail(c, GR_EINVAL, "bad batch");
+  if (!c->d_pins) return fail(c, GR_ESTATE, "gr_set_nets must come first");
+  if (c->prop || c->collect)
+    return fail(c, GR_ESTATE, "the allocation-free entry point routes with exhaustive costing and no sample capture");
+  if (!c->scratch) return fail(c, GR_ESTATE, "no static scratch address for this device");
+  (void)hipSetDevice(c->dev);
+  Sched st;
+  HostSched& h = c->hst;
+  int32_t ofs[2] = {0, n};
+  int rc = host_sched(c, st, h, 1, ofs, net_ids);
+  if (rc) return rc;
+  // carve the static array: descriptors, chunk owners, outputs, pair results
+  const size_t npos = h.slot.size(), nch = h.chunk.size();
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return at;
+  };
+  const size_t o_pinofs = take(8 * npos), o_slot = take(8 * npos), o_pair = take(8 * npos);
+  const size_t o_deg = take(4 * npos), o_list = take(4 * npos), o_cost = take(4 * npos);
+  const size_t o_chunk = take(4 * nch);
+  const size_t o_edges = take(8 * (size_t)h.slots), o_lpat = take(4 * (size_t)h.slots);
+  const size_t o_pc = take(4 * (size_t)h.maxP), o_pp = take(2 * (size_t)h.maxP);
+  if (off > kScratchBytes)
+    return fail(c, GR_ERANGE,
+                "batch needs " + std::to_string(off) + " bytes of device scratch; the allocation-free entry point has " +
+                    std::to_string(kScratchBytes) + " (use gr_route_batch)");
+  unsigned char* base = (unsigned char*)c->scratch;
+  st.ext = true;
+  st.d_pinofs = (int64_t*)(base + o_pinofs);
+  st.d_slot = (int64_t*)(base + o_slot);
+  st.d_pair = (int64_t*)(base + o_pair);
+  st.d_deg = (int32_t*)(base + o_deg);
+  st.d_list = (int32_t*)(base + o_list);
+  st.d_cost = (float*)(base + o_cost);
+  st.d_chunk = (int32_t*)(base + o_chunk);
+  st.d_edges = (int64_t*)(base + o_edges);
+  st.d_lpat = (int32_t*)(base + o_lpat);
+  st.x_pair_cost = (float*)(base + o_pc);
+  st.x_pair_pat = (int16_t*)(base + o_pp);
+  if ((rc = upload_sched(c, st, h))) return rc;
+  // per-kernel timing records events, which may allocate: not on this path
+  const bool timing = c->timing;
+  c->timing = false;
+  rc = enqueue_batch(c, st, 0);
+  c->timing = timing;
+  if (rc) return rc;
+  const int64_t slots = st.slots;
+  if (out_edges && slots)
+    HIPC(c, hipMemcpyAsync(out_edges, st.d_edges, sizeof(int64_t) * slots, hipMemcpyDeviceToHost, c->stream));
+  if (out_lpat && slots)
+    HIPC(c, hipMemcpyAsync(out_lpat, st.d_lpat, sizeof(int32_t) * slots, hipMemcpyDeviceToHost, c->stream));
+  if (out_net_cost && n)
+    HIPC(c, hipMemcpyAsync(out_net_cost, st.d_cost, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int64_t gr_static_scratch_bytes(void) { return (int64_t)kScratchBytes; }
+
 int gr_set_schedule(gr_ctx* c, int32_t n_batches, const int32_t* batch_ofst, const int32_t* net_ids) {
   if (!c || n_batches < 0 || !batch_ofst || (batch_ofst[n_batches] && !net_ids)) return fail(c, GR_EINVAL, "bad schedule");
   if (!c->d_pins) return fail(c, GR_ESTATE, "gr_set_nets must come first");
@@ -588,6 +709,8 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
     if (s.exec) {
       HIPC(c, hipGraphLaunch(s.exec, c->stream));
       c->last_P = s.meta.back().P;
+      c->last_pair_cost = c->d_pair_cost;
+      c->last_pair_pat = c->d_pair_pat;
       return 0;
     }
   }
@@ -660,8 +783,10 @@ int64_t gr_last_pairs(const gr_ctx* c) { return c ? c->last_P : 0; }
 int gr_get_pair_costs(gr_ctx* c, float* cost, int16_t* pat) {
   if (!c) return fail(c, GR_EINVAL, "null context");
   HIPC(c, hipStreamSynchronize(c->stream));
-  if (cost && c->last_P) HIPC(c, hipMemcpy(cost, c->d_pair_cost, sizeof(float) * c->last_P, hipMemcpyDeviceToHost));
-  if (pat && c->last_P) HIPC(c, hipMemcpy(pat, c->d_pair_pat, sizeof(int16_t) * c->last_P, hipMemcpyDeviceToHost));
+  if (cost && c->last_P)
+    HIPC(c, hipMemcpy(cost, c->last_pair_cost, sizeof(float) * c->last_P, hipMemcpyDeviceToHost));
+  if (pat && c->last_P)
+    HIPC(c, hipMemcpy(pat, c->last_pair_pat, sizeof(int16_t) * c->last_P, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -127,6 +127,30 @@ int gr_route_batch(gr_ctx* ctx, const int32_t* net_ids, int32_t n,
                    int64_t* out_edges, int32_t* out_lpat, float* out_net_cost);
 
 /*
+ * Allocation-free variant of gr_route_batch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs and results as gr_route_batch, bit for bit, but it
+ * calls no hipMalloc/hipFree: every device buffer the batch needs beyond the
+ * context's grid state and nets (per-net descriptors, the pair-owner table,
+ * per-pair minimum costs and patterns, output slots) lives in one static
+ * device array of GR_STATIC_SCRATCH_BYTES per GPU, whose address the library
+ * takes once per device when the first context is created there.  A batch
+ * whose buffers do not fit (6 B per pin pair + 4 B per 64 pairs + 12 B per
+ * pin + 36 B per net, each part rounded up to 256 B) gets GR_ERANGE before any device work.
+ * Exhaustive costing only: GR_ESTATE while a learned proposal or sample
+ * capture is set.  Per-kernel timing is not recorded for these calls.
+ * Every call on one GPU uses that GPU's one array, which keeps the last
+ * call's contents (nothing in it needs clearing: a call writes every byte it
+ * reads); calls that overlap on one GPU, from any context, race.
+ */
+#define GR_STATIC_SCRATCH_BYTES (64ll << 20)
+int gr_route_batch_static(gr_ctx* ctx, const int32_t* net_ids, int32_t n,
+                          int64_t* out_edges, int32_t* out_lpat, float* out_net_cost);
+/* Size of that array in bytes (GR_STATIC_SCRATCH_BYTES). */
+int64_t gr_static_scratch_bytes(void);
+
+/*
  * Keep a whole batch schedule resident on the device: batch b covers
  * net_ids[batch_ofst[b] .. batch_ofst[b+1]).  Replaces the batcher's per-batch
  * uploads and scans (part_net/net_part.hpp:345-369).  A routed net may have
