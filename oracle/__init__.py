@@ -42,6 +42,8 @@ _SIGS = {
     "orc_set_tree_mode": (None, [_P, ctypes.c_int]),
     "orc_route_batch": (_I64, [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]),
     "orc_last_pairs": (_I64, [_P]),
+    "orc_partition": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int,
+                                     _P, _P, ctypes.c_int, _P]),
     "orc_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
 }
 
@@ -86,6 +88,23 @@ def boruvka(n, costs, true_mst=False):
     edges = [(int(v >> 15), int(v & 0x7FFF)) for v in out[: n - 1]]
     assert out[n - 1] == -1
     return edges, nc.value, rounds
+
+
+def partition(X, Y, pin_pfx, pin_enc, parts=4, tile=32, trace_batch=-1):
+    """Batches of the reference's batch mode (net_part.hpp:167-339):
+    (batch_ofst, net_ids[, per-net bucket in trace_batch]).  tile = 0 tests
+    boxes by brute force."""
+    pfx = np.ascontiguousarray(pin_pfx, np.int64)
+    pins = np.ascontiguousarray(pin_enc, np.int32)
+    n = len(pfx) - 1
+    ofst = np.empty(n + 1, np.int32)
+    ids = np.empty(max(n, 1), np.int32)
+    trace = np.empty(max(n, 1), np.int32)
+    nb = lib().orc_partition(X, Y, n, _ptr(pfx), _ptr(pins), parts, tile, _ptr(ofst), _ptr(ids), trace_batch,
+                             _ptr(trace))
+    assert nb >= 0
+    res = (ofst[:nb + 1].copy(), ids[:n].copy())
+    return res + (trace[:n].copy(),) if trace_batch >= 0 else res
 
 
 def cw_exp(x):

@@ -73,19 +73,25 @@ def features(X, Y, Z, rem, lut, src, dst, hidden=256):
     return bf16(out)
 
 
-def forward(p, x):
-    """Logits (fp32) and bf16 activations of rows x."""
-    xb = bf16(x)
-    W1, W2, W3 = bf16(p["W1"]), bf16(p["W2"]), bf16(p["W3"])
-    h1 = bf16(np.maximum(xb @ W1 + p["b1"], 0))
-    h2 = bf16(np.maximum(h1 @ W2 + p["b2"], 0))
-    logits = (h2 @ W3 + p["b3"]).astype(np.float32)
-    return logits, (xb, h1, h2)
+def forward(p, x, exact=False):
+    """Logits (fp32) and bf16 activations of rows x.  exact=True: float64
+    throughout with no bf16 rounding (for checking gradients against finite
+    differences)."""
+    r = (lambda v: np.asarray(v, np.float64)) if exact else bf16
+    xb = r(x)
+    W1, W2, W3 = r(p["W1"]), r(p["W2"]), r(p["W3"])
+    h1 = r(np.maximum(xb @ W1 + p["b1"], 0))
+    h2 = r(np.maximum(h1 @ W2 + p["b2"], 0))
+    logits = h2 @ W3 + p["b3"]
+    return (logits if exact else logits.astype(np.float32)), (xb, h1, h2)
 
 
-def gradient(p, x, y, n_out):
-    """Mean cross-entropy over the first n_out logits and its gradient."""
-    logits, (xb, h1, h2) = forward(p, x)
+def gradient(p, x, y, n_out, exact=False):
+    """Mean cross-entropy over the first n_out logits and its gradient (flat,
+    in parameter order W1, b1, W2, b2, W3, b3).  exact=True: float64, no bf16
+    rounding."""
+    r = (lambda v: np.asarray(v, np.float64)) if exact else bf16
+    logits, (xb, h1, h2) = forward(p, x, exact)
     n = x.shape[0]
     l = logits[:, :n_out].astype(np.float64)
     l = l - l.max(axis=1, keepdims=True)
@@ -95,16 +101,17 @@ def gradient(p, x, y, n_out):
     d = sm.copy()
     d[np.arange(n), y] -= 1.0
     dL = np.zeros_like(logits)
-    dL[:, :n_out] = bf16((d / n).astype(np.float32))
-    W2, W3 = bf16(p["W2"]), bf16(p["W3"])
+    dL[:, :n_out] = r(d / n) if exact else r((d / n).astype(np.float32))
+    W2, W3 = r(p["W2"]), r(p["W3"])
     g = {}
     g["W3"] = h2.T @ dL
     g["b3"] = dL.sum(axis=0)
-    dz2 = bf16((dL @ W3.T) * (h2 > 0))
+    dz2 = r((dL @ W3.T) * (h2 > 0))
     g["W2"] = h1.T @ dz2
     g["b2"] = dz2.sum(axis=0)
-    dz1 = bf16((dz2 @ W2.T) * (h1 > 0))
+    dz1 = r((dz2 @ W2.T) * (h1 > 0))
     g["W1"] = xb.T @ dz1
     g["b1"] = dz1.sum(axis=0)
-    flat = np.concatenate([g[k].astype(np.float32).ravel() for k in ("W1", "b1", "W2", "b2", "W3", "b3")])
+    ft = np.float64 if exact else np.float32
+    flat = np.concatenate([g[k].astype(ft).ravel() for k in ("W1", "b1", "W2", "b2", "W3", "b3")])
     return flat, loss

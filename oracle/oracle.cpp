@@ -679,6 +679,141 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
   return touched;
 }
 
+// ---------------------------------------------------------------------------
+// Batch generation: NET_PART::part_nets in its batch mode
+// (part_net/net_part.hpp:236-262: nets first sorted by (minx, maxx); then per
+// batch part_xdim, :167-235, and the compaction of :326-337).
+//
+// One batch:
+//   * targ = ceil(remaining / parts)                                   :168
+//   * the remaining nets are sorted by maxx                            :186-188
+//   * walking them in that order, a net joins the current x-bucket when its
+//     minx lies right of the bucket cut (cut = maxx of the net that closed
+//     the previous bucket, -1 at first); a bucket closes once it holds targ
+//     nets; nets left of the cut join no bucket this batch                :191-202
+//   * each bucket lists its nets in reverse walking order (the fill of
+//     pick_ids from the back, :207-212), then is sorted by half perimeter
+//     (W + H, :178-182, :215-218)
+//   * in that order a net is taken when its closed 2-D box meets no box
+//     already taken in its bucket (Boost rtree "intersects", :158-166, 219-226)
+//   * the taken nets form the batch; the others keep their relative order
+//     at the front of the list for the next batch (:326-337)
+// Ties of the three sorts follow std::sort on the same sequence with the same
+// comparators (unstable: the tie order is the library's, as in the reference
+// built with the same g++).  The compaction loop of :326-337 keeps the nets not
+// taken in their relative order (its scan holds every taken net between the
+// write and read positions), which is std::stable_partition.  The batch's own
+// order is (pin count, net id): the reference appends under omp critical and
+// sorts by pin count with std::sort (:227-230, :347-349), which fixes no order.
+//
+// Box tests here use a uniform grid of 32x32-GCELL tiles listing the boxes
+// taken so far that overlap each tile, and the exact closed-box test on them;
+// brute force (every taken box) when tile == 0.
+namespace {
+struct PBox { int minx, maxx, miny, maxy; };
+inline bool boxes_meet(const PBox& a, const PBox& b) {
+  return !(a.maxx < b.minx || b.maxx < a.minx || a.maxy < b.miny || b.maxy < a.miny);
+}
+struct TakenBoxes {
+  int tile, tx, ty;
+  std::vector<PBox> all;
+  std::vector<std::vector<int>> bins;
+  TakenBoxes(int X, int Y, int t) : tile(t) {
+    tx = t ? (X + t - 1) / t : 1;
+    ty = t ? (Y + t - 1) / t : 1;
+    bins.assign((size_t)tx * ty, {});
+  }
+  bool meets(const PBox& b) const {
+    if (!tile) {
+      for (const PBox& o : all) if (boxes_meet(o, b)) return true;
+      return false;
+    }
+    for (int j = b.miny / tile; j <= b.maxy / tile; ++j)
+      for (int i = b.minx / tile; i <= b.maxx / tile; ++i)
+        for (int k : bins[(size_t)j * tx + i]) if (boxes_meet(all[k], b)) return true;
+    return false;
+  }
+  void add(const PBox& b) {
+    const int k = (int)all.size();
+    all.push_back(b);
+    if (!tile) return;
+    for (int j = b.miny / tile; j <= b.maxy / tile; ++j)
+      for (int i = b.minx / tile; i <= b.maxx / tile; ++i) bins[(size_t)j * tx + i].push_back(k);
+  }
+};
+}  // namespace
+
+// Returns the number of batches; batch_ofst[nb+1], net_ids[n_nets].  With
+// trace_batch >= 0, trace_bucket[n_nets] gets each net's bucket in that batch
+// (-1: in no bucket, -2: taken by an earlier batch).
+int orc_partition(int X, int Y, int n_nets, const int64_t* pin_pfx, const int32_t* pins, int parts, int tile,
+                  int32_t* batch_ofst, int32_t* net_ids, int trace_batch, int32_t* trace_bucket) {
+  if (X <= 0 || Y <= 0 || n_nets < 0 || parts < 1) return -1;
+  std::vector<PBox> bx(n_nets);
+  std::vector<int> deg(n_nets);
+  const int64_t XY = (int64_t)X * Y;
+  for (int i = 0; i < n_nets; ++i) {
+    PBox b{X, -1, Y, -1};
+    for (int64_t p = pin_pfx[i]; p < pin_pfx[i + 1]; ++p) {
+      const int64_t v = pins[p], z = v / XY, r = v - z * XY;
+      const int x = (z & 1) ? (int)(r / Y) : (int)(r % X);
+      const int y = (z & 1) ? (int)(r % Y) : (int)(r / X);
+      b.minx = std::min(b.minx, x); b.maxx = std::max(b.maxx, x);
+      b.miny = std::min(b.miny, y); b.maxy = std::max(b.maxy, y);
+    }
+    bx[i] = b;
+    deg[i] = (int)(pin_pfx[i + 1] - pin_pfx[i]);
+  }
+  std::vector<int> order(n_nets);
+  for (int i = 0; i < n_nets; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {  // net_part.hpp:244-249, 259-262
+    if (bx[a].minx == bx[b].minx) return bx[a].maxx < bx[b].maxx;
+    return bx[a].minx < bx[b].minx;
+  });
+  std::vector<char> done(n_nets, 0);
+  int remaining = n_nets, nb = 0, out = 0;
+  batch_ofst[0] = 0;
+  while (remaining > 0) {
+    const int targ = (remaining + parts - 1) / parts;
+    std::sort(order.begin(), order.begin() + remaining, [&](int a, int b) { return bx[a].maxx < bx[b].maxx; });
+    std::vector<std::vector<int>> bucket(1);
+    int cut = -1;
+    for (int i = 0; i < remaining; ++i) {
+      const int id = order[i];
+      if (bx[id].minx > cut) bucket.back().push_back(id);
+      if ((int)bucket.back().size() == targ) {
+        cut = bx[id].maxx;
+        bucket.emplace_back();
+      }
+    }
+    if (trace_batch == nb)
+      for (int i = 0; i < n_nets; ++i) trace_bucket[i] = done[i] ? -2 : -1;
+    std::vector<int> batch;
+    for (size_t k = 0; k < bucket.size(); ++k) {
+      std::vector<int>& list = bucket[k];
+      std::reverse(list.begin(), list.end());
+      std::sort(list.begin(), list.end(), [&](int a, int b) {
+        return (bx[a].maxx - bx[a].minx + 1) + (bx[a].maxy - bx[a].miny + 1) <
+               (bx[b].maxx - bx[b].minx + 1) + (bx[b].maxy - bx[b].miny + 1);
+      });
+      TakenBoxes taken(X, Y, tile);
+      for (int id : list) {
+        if (trace_batch == nb) trace_bucket[id] = (int32_t)k;
+        if (taken.meets(bx[id])) continue;
+        taken.add(bx[id]);
+        batch.push_back(id);
+        done[id] = 1;
+      }
+    }
+    std::stable_partition(order.begin(), order.begin() + remaining, [&](int id) { return !done[id]; });
+    remaining -= (int)batch.size();
+    std::sort(batch.begin(), batch.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+    for (int id : batch) net_ids[out++] = id;
+    batch_ofst[++nb] = out;
+  }
+  return nb;
+}
+
 int64_t orc_last_pairs(void* h) { return (int64_t)((Ctx*)h)->last_cost.size(); }
 
 int orc_get_pair_costs(void* h, float* cost, int16_t* pat) {

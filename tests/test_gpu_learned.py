@@ -289,3 +289,93 @@ def test_always_fallback_full_workload(gr, gpu):
     pairs, fb = r.proposal_stats()
     assert pairs == 2 * int((np.diff(pfx) * (np.diff(pfx) - 1) // 2).sum()) and fb == pairs
     r.set_proposal(None)
+
+
+def test_trained_topk_full_workload_sampled(gr, orc, gpu):
+    """The learned workload at its full size: 800x800x10, 150,000 nets
+    requested, a model trained on exhaustive routing of two other seeds,
+    top-k in {4, 8} with the 0.6 confidence threshold.  On sampled pairs of
+    a bulk, a middle and a tail batch, each pair's (cost, pattern) equals the
+    oracle's costing at the GPU's own state before the batch: the (cost, p)
+    minimum over the model's top-k candidates when the top-1 softmax
+    probability is at least 0.6, else the exhaustive minimum.  Both kinds of
+    pair must occur."""
+    X, Y, Z = 800, 800, 10
+    Z2 = (Z + 1) // 2
+    cap = gr.gen_grid(2, 0, X, Y, Z)
+    r = gr.Router(X, Y, Z, cap)
+    m = gr.Model(r, hidden=256, seed=0, max_samples=1 << 18)
+    m.collect(1000)
+    for seed in (1, 2):
+        pf, pn = gr.gen_nets(2, seed, X, Y, Z, 150000)
+        of, ii = gr.partition_nets(X, Y, Z, pf, pn, 4)
+        r.set_nets(pf, pn)
+        r.set_schedule(of, ii)
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+    m.collect(0)
+    for e in range(3):
+        m.train(epochs=1, batch=1024, lr=0.05, seed=e)
+    pfx, pins = gr.gen_nets(2, 0, X, Y, Z, 150000)
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    o = orc.Oracle(X, Y, Z, cap)
+    rng = np.random.default_rng(9)
+    nb = len(ofst) - 1
+    seen_fb = seen_topk = 0
+    for k in (4, 8):
+        r.set_proposal(m, k=k, threshold=0.6)
+        for b in (0, nb // 3, nb - 3):
+            print(f"trained top-{k} batch {b}", flush=True)
+            r.reset_demand()
+            if b:
+                r.route_schedule(0, b)
+            r.sync()
+            o.set_rem(cap - r.demand())
+            batch = ids[ofst[b]:ofst[b + 1]]
+            deg = np.diff(pfx)[batch]
+            P = deg * (deg - 1) // 2
+            first = np.concatenate([[0], np.cumsum(P)])
+            tot = int(first[-1])
+            sample = np.sort(rng.choice(tot, size=min(tot, 3000), replace=False))
+            owner = np.searchsorted(first, sample, side="right") - 1
+            src, dst = [], []
+            for w, q in zip(sample, owner):
+                o_ = int(w - first[q])
+                j = int((1 + np.sqrt(1 + 8 * o_)) // 2)
+                while j * (j - 1) // 2 > o_:
+                    j -= 1
+                while (j + 1) * j // 2 <= o_:
+                    j += 1
+                kk = o_ - j * (j - 1) // 2
+                p = pins[pfx[batch[q]]:pfx[batch[q] + 1]]
+                src.append(int(p[j])); dst.append(int(p[kk]))
+            logits = m.forward(m.features(np.array(src, np.int32), np.array(dst, np.int32)))[:, :m.n_out]
+            r.route_schedule(b, 1)
+            cost, pat = r.pair_costs()
+            for i, w in enumerate(sample):
+                lg = logits[i].astype(np.float64)
+                p1 = 1.0 / np.exp(lg - lg.max()).sum()
+                order = sorted(range(m.n_out), key=lambda q: (-logits[i, q], q))[:k]
+                top = None
+                for q in order:
+                    c = o.pattern_cost(src[i], dst[i], q + Z2)
+                    if c is not None and (top is None or (np.float32(c), q + Z2) < top):
+                        top = (np.float32(c), q + Z2)
+                pp, cc = o.best_pattern(src[i], dst[i])
+                full = (np.float32(cc), pp)
+                got = (np.float32(cost[w]), int(pat[w]))
+                if abs(p1 - 0.6) < 1e-4:
+                    assert got in (top, full), (k, b, i)
+                elif p1 < 0.6 or top is None:
+                    assert got == full, (k, b, i, got, full)
+                    seen_fb += 1
+                else:
+                    assert got == top, (k, b, i, got, top)
+                    seen_topk += 1
+    r.set_proposal(None)
+    print(f"sampled pairs: {seen_topk} from top-k, {seen_fb} fell back", flush=True)
+    assert seen_fb > 0 and seen_topk > 0
+    m.close()
