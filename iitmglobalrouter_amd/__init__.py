@@ -15,7 +15,7 @@ import numpy as np
 __all__ = [
     "load_library", "RouterError", "Router", "Model", "Tuning", "Design", "route_cost", "gen_grid", "gen_nets",
     "partition_nets", "FMT_FIRST_ACCESS_POINT", "FMT_NEW", "WRITE_DBU", "WRITE_MERGE", "WRITE_PATCH",
-    "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED",
+    "COST_REF_BIT", "COST_REF_INC", "COST_LIT_EXP", "COMMIT_REF_QUIRK", "COMMIT_FIXED", "COMMIT_PREFIX",
     "WORKLOADS", "device_count", "TREE_ROOT_PAIR", "TREE_TRUE_MST",
 ]
 
@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libiitmgr.so")
 
 COST_REF_BIT, COST_REF_INC, COST_LIT_EXP = 0, 1, 2
-COMMIT_REF_QUIRK, COMMIT_FIXED = 0, 1
+COMMIT_REF_QUIRK, COMMIT_FIXED, COMMIT_PREFIX = 0, 1, 2
 TREE_ROOT_PAIR, TREE_TRUE_MST = 0, 1
 K_LPAT, K_TREE, K_REFRESH = 0, 1, 2
 
@@ -86,6 +86,7 @@ _SIGS = {
     "gr_get_demand": (ctypes.c_int, [_P, _P]),
     "gr_get_prefix": (ctypes.c_int, [_P, _P]),
     "gr_last_pairs": (_I64, [_P]),
+    "gr_get_via_terms": (ctypes.c_int, [_P, _P]),
     "gr_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
     "gr_get_lut": (ctypes.c_int, [_P, _P]),
     "gr_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -453,6 +454,12 @@ class Router:
     def prefix(self):
         out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
         self._c(self.lib.gr_get_prefix(self.h, _ptr(out)))
+        return out
+
+    def via_terms(self):
+        """Per-cell via cost term (BIT: Q(pos) - Q(pos-1); non-BIT: the layer's via cost)."""
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        self._c(self.lib.gr_get_via_terms(self.h, _ptr(out)))
         return out
 
     def pair_costs(self):

@@ -78,6 +78,18 @@ __host__ __device__ inline float cost_f32(int model, float r) {
   return v;
 }
 
+// Increment of the base cost f(r) = e^(-r/2) when r drops by d, for the
+// non-BIT mode: f(r - d) - f(r) with r - d in fp32 (get_gpu_of_cost_inc,
+// util.hpp:127-129), each f rounded to fp32 from cw_exp, saturated at 1e30
+// (an infinite or NaN difference saturates too).  d = 1 is a wire cell's
+// cost, d = 0.5 half a via charge.
+__host__ __device__ inline float inc_cost(float r, float d) {
+  const float rd = r - d;
+  float v = (float)cw_exp(-0.5 * (double)rd) - (float)cw_exp(-0.5 * (double)r);
+  if (!(v < 1e30f)) v = 1e30f;
+  return v;
+}
+
 // Orderable key of an fp32 cost: -0 and +0 map to the same key, negatives
 // below positives, so (key << 32 | index) orders lexicographically by
 // (cost, index).

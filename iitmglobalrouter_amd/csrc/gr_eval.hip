@@ -193,6 +193,7 @@ extern "C" {
 
 int gr_evaluate(gr_ctx* c, gr_eval_layer* layers, gr_eval_total* total) {
   if (!c) return fail(c, GR_EINVAL, "null context");
+  if (c->prefix) return fail(c, GR_ESTATE, "the evaluator reports BIT-mode passes (integer demand)");
   Sched& s = c->sched;
   if (s.meta.empty()) return fail(c, GR_ESTATE, "gr_set_schedule and a routed pass must come first");
   (void)hipSetDevice(c->dev);

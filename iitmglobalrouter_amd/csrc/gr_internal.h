@@ -79,6 +79,8 @@ struct gr_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   grd::Geo g{};
   int model = 0, quirk = 1, M = 0;
+  int prefix = 0;  // GR_COMMIT_PREFIX: the reference's non-BIT mode (prefix costing, difference-array commit)
+  int32_t* d_vhalf = nullptr;  // non-BIT mode: via half charges per cell of the current batch
   int true_mst = 0;  // GR_TREE_TRUE_MST
   int64_t cs = 0;
   float lut[grd::kLut];

@@ -252,6 +252,8 @@ struct TreeArgs {
   int32_t quirk;
   int32_t M;  // run_stamp row stride: max(X, Y)
   int32_t true_mst;  // record each accepted component's minimum pair instead of the root pair
+  int32_t prefix;    // non-BIT commit: half-open wire marks and via half charges
+  int32_t* vhalf;    // non-BIT commit: via half charges per cell of this batch
 };
 
 // The edge an accepted candidate of component root r records: the root pair
@@ -298,9 +300,68 @@ __device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int 
 // A quirk cell whose enc would be negative (possible only when X*Y < Z) lies
 // outside the grid: the reference writes before its array there; it is
 // skipped here.
+// Non-BIT commit of one edge (MST_COMMIT, mst_commit.hpp:114-251): each
+// wire marks the cells [min, max) of its run (the difference array's +1 at
+// the low end and -1 at the high end, gpu_commit_wire :68-113), the first on
+// a, the second on b' whenever b' > 0 (gpu_add_wires :188-251); via stacks
+// z1 -> a at the source, a -> b' after the first wire when b' > 0, then to
+// z2, each over layers [min, max) (gpu_add_vias :114-186), charge half units
+// (gpu_commit_via :17-61): 2 to the cell at run position 0, 2 to the
+// previous cell at the last position, else 1 to each.  The refresh applies
+// them: -1.0 once per wired cell, then -0.5 per half.
+__device__ __forceinline__ void prefix_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
+  if (lo >= hi) return;
+  const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
+  const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
+  for (int t = lo; t < hi; ++t) A.mark[base + t] = 1;
+  mark_run(A, z, line);
+}
+__device__ __forceinline__ void prefix_via(const TreeArgs& A, int za, int zb, int x, int y) {
+  const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
+  for (int i = lo; i < hi; ++i) {
+    const int pos = (i & 1) ? y : x, L = (i & 1) ? A.g.Y : A.g.X;
+    const uint32_t c = enc(A.g, i, x, y);
+    if (pos == 0) atomicAdd(&A.vhalf[c], 2);
+    else if (pos == L - 1) atomicAdd(&A.vhalf[c - 1], 2);
+    else { atomicAdd(&A.vhalf[c - 1], 1); atomicAdd(&A.vhalf[c], 1); }
+    mark_run(A, i, (i & 1) ? x : y);
+  }
+}
+__device__ void commit_edge_prefix(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
+  const Geo& g = A.g;
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, src, z1, x1, y1);
+  dec(g, dst, z2, x2, y2);
+  const int a = p / g.Z2, b = p - a * g.Z2;
+  const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+  // wires
+  int x = x1, y = y1;
+  if (a & 1) { prefix_wire(A, a, x, min(y, y2), max(y, y2)); y = y2; }
+  else       { prefix_wire(A, a, y, min(x, x2), max(x, x2)); x = x2; }
+  if (bp > 0) {
+    if (bp & 1) prefix_wire(A, bp, x, min(y, y2), max(y, y2));
+    else        prefix_wire(A, bp, y, min(x, x2), max(x, x2));
+  }
+  // vias
+  x = x1; y = y1;
+  prefix_via(A, z1, a, x, y);
+  int z = a;
+  if (z & 1) y = y2; else x = x2;
+  if (bp > 0) {
+    prefix_via(A, z, bp, x, y);
+    z = bp;
+    if (z & 1) y = y2; else x = x2;
+  }
+  if (z != z2) prefix_via(A, z, z2, x, y);
+}
+
 __device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
   const Geo& g = A.g;
   if (p < g.Z2 || p >= g.Z * g.Z2) return;  // no valid pattern (cannot happen for Z >= 3)
+  if (A.prefix) {
+    commit_edge_prefix(A, src, dst, p);
+    return;
+  }
   int z1, x1, y1, z2, x2, y2;
   dec(g, src, z1, x1, y1);
   dec(g, dst, z2, x2, y2);

@@ -18,6 +18,7 @@
 #include "../../include/iitmgr.h"
 #include "gr_internal.h"
 #include "gr_kernels.h"
+#include "gr_prefix.h"
 
 using namespace grd;
 
@@ -106,10 +107,14 @@ struct KTimer {
 };
 
 size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
+size_t refresh_prefix_lds(const gr_ctx* c) { return (3 * (size_t)c->M + 128) * sizeof(float); }
 int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
 
 void launch_refresh(gr_ctx* c, int all) {
-  if (c->d_remf)
+  if (c->prefix)
+    hipLaunchKernelGGL(k_refresh_prefix, dim3(refresh_grid(c)), dim3(256), refresh_prefix_lds(c), c->stream, c->g, c->M,
+                       c->d_remf, c->d_mark, c->d_vhalf, c->d_pfx, c->d_vterm, c->d_run_stamp, c->d_ctr, all);
+  else if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
                        c->d_vterm, (int16_t*)nullptr);
@@ -265,6 +270,24 @@ int lpat_block_for(int tuned, int64_t P) {
   return 64;
 }
 
+typedef void (*PrefixFn)(Geo, const float*, const float*, const int32_t*, const int64_t*, const int64_t*,
+                         const int32_t*, int, int64_t, float*, int16_t*, int32_t*);
+PrefixFn prefix_kernel(int Z) {
+  switch (Z) {
+    case 3: return k_lpat_prefix<3>;
+    case 4: return k_lpat_prefix<4>;
+    case 5: return k_lpat_prefix<5>;
+    case 6: return k_lpat_prefix<6>;
+    case 7: return k_lpat_prefix<7>;
+    case 8: return k_lpat_prefix<8>;
+    case 9: return k_lpat_prefix<9>;
+    case 10: return k_lpat_prefix<10>;
+    case 11: return k_lpat_prefix<11>;
+    case 12: return k_lpat_prefix<12>;
+    default: return nullptr;
+  }
+}
+
 LpatFn lane_kernel(int Z) {
   switch (Z) {
     case 3: return k_lpat_lane<3>;
@@ -357,6 +380,19 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     KTimer t(c, GR_K_LPAT);
     const int rc = mlp_enqueue_costing(c, s, m);
     if (rc) return rc;
+  } else if (m.P > 0 && c->prefix) {
+    KTimer t(c, GR_K_LPAT);
+    PrefixFn fn = c->tune.lpat_wave ? nullptr : prefix_kernel(c->g.Z);
+    if (fn) {
+      const int lb = lpat_block_for(c->tune.lpat_block, m.P);
+      hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
+                         c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n,
+                         m.P, pcost, ppat, c->d_ctr);
+    } else {
+      hipLaunchKernelGGL(k_lpat_prefix_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
+                         c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n,
+                         m.P, pcost, ppat, c->d_ctr);
+    }
   } else if (m.P > 0) {
     KTimer t(c, GR_K_LPAT);
     LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
@@ -395,6 +431,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.quirk = c->quirk;
   A.M = c->M;
   A.true_mst = c->true_mst;
+  A.prefix = c->prefix;
+  A.vhalf = c->d_vhalf;
   {
     KTimer t(c, GR_K_TREE);
     const int rc = launch_trees(c, s, m, A);
@@ -450,8 +488,10 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
     return fail(nullptr, GR_ERANGE, "grid must have 1 <= X, Y <= 8192 and 3 <= Z <= 32");
   const int64_t cs = (int64_t)d->X * d->Y * d->Z;
   if (cs >= (1ll << 31)) return fail(nullptr, GR_ERANGE, "grid has 2^31 or more cells");
-  if (d->cost_model < 0 || d->cost_model > 2 || d->commit_mode < 0 || d->commit_mode > 1)
+  if (d->cost_model < 0 || d->cost_model > 2 || d->commit_mode < 0 || d->commit_mode > 2)
     return fail(nullptr, GR_EINVAL, "unknown cost model or commit mode");
+  if (d->commit_mode == GR_COMMIT_PREFIX && d->cost_model != GR_COST_REF_INC)
+    return fail(nullptr, GR_EINVAL, "the non-BIT mode costs the reference's increments: use GR_COST_REF_INC");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device || d->device < 0)
     return fail(nullptr, GR_ENODEV, "no HIP device");
@@ -469,6 +509,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   c->g.XY = (uint32_t)d->X * (uint32_t)d->Y;
   c->model = d->cost_model;
   c->quirk = d->commit_mode == GR_COMMIT_REF_QUIRK;
+  c->prefix = d->commit_mode == GR_COMMIT_PREFIX;
   c->cs = cs;
   c->M = std::max(d->X, d->Y);
   build_lut(c->model, c->lut);
@@ -483,7 +524,8 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
       (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_vterm, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
-      (capf && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs))))) {
+      ((capf || c->prefix) && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs)))) ||
+      (c->prefix && (rc = dalloc(c, &c->d_vhalf, cs)))) {
     g_err = c->err;
     return bail(rc);
   }
@@ -491,10 +533,18 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   bool ok = hipFuncSetAttribute((const void*)k_refresh<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)refresh_lds(c)) == hipSuccess &&
             hipFuncSetAttribute((const void*)k_refresh<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)refresh_lds(c)) == hipSuccess;
+                                (int)refresh_lds(c)) == hipSuccess &&
+            hipFuncSetAttribute((const void*)k_refresh_prefix, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)refresh_prefix_lds(c)) == hipSuccess;
+  if (c->prefix && refresh_prefix_lds(c) > c->lds_max) ok = false;
   if (cap) {
     ok = ok && hipMemcpy(c->d_cap0, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(c->d_rem, cap, sizeof(int32_t) * cs, hipMemcpyHostToDevice) == hipSuccess;
+    if (c->prefix) {  // the non-BIT state is the fp32 remaining capacity
+      std::vector<float> f(cap, cap + cs);
+      ok = ok && hipMemcpy(c->d_capf0, f.data(), sizeof(float) * cs, hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(c->d_remf, f.data(), sizeof(float) * cs, hipMemcpyHostToDevice) == hipSuccess;
+    }
   } else {
     // fp32 grid: the integer state starts at 0 and counts commits downwards,
     // so cap0 - rem is still the demand
@@ -511,6 +561,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
+      (c->d_vhalf && hipMemset(c->d_vhalf, 0, sizeof(int32_t) * cs) != hipSuccess) ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
     g_err = "device set-up failed";
     return bail(GR_ENODEV);
@@ -542,7 +593,7 @@ void gr_destroy(gr_ctx* c) {
   mlp_detach_all(c);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_mark); (void)hipFree(c->d_run_stamp);
   (void)hipFree(c->d_pfx); (void)hipFree(c->d_vterm); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
-  (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0);
+  (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0); (void)hipFree(c->d_vhalf);
   (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
   (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
   (void)hipFree(c->d_pin_pfx); (void)hipFree(c->d_pins);
@@ -762,6 +813,8 @@ int gr_get_remaining_f32(gr_ctx* c, float* out) {
 
 int gr_get_demand(gr_ctx* c, int32_t* out) {
   if (!c || !out) return fail(c, GR_EINVAL, "null argument");
+  if (c->prefix)
+    return fail(c, GR_ESTATE, "the non-BIT mode's state is fp32 remaining capacity in half units: gr_get_remaining_f32");
   (void)hipSetDevice(c->dev);
   std::vector<int32_t> cap0(c->cs);
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -775,6 +828,13 @@ int gr_get_prefix(gr_ctx* c, float* out) {
   if (!c || !out) return fail(c, GR_EINVAL, "null argument");
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(out, c->d_pfx, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gr_get_via_terms(gr_ctx* c, float* out) {
+  if (!c || !out) return fail(c, GR_EINVAL, "null argument");
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(out, c->d_vterm, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
   return 0;
 }
 

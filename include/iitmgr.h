@@ -54,6 +54,18 @@ extern "C" {
 /* commit modes */
 #define GR_COMMIT_REF_QUIRK 0 /* reproduce the unguarded first wire, mst_bit_commit.hpp:216 */
 #define GR_COMMIT_FIXED 1     /* mark only the cells of the route */
+/* The reference's non-BIT mode (-bit_pmodel off), with GR_COST_REF_INC only:
+ * costing from an exclusive prefix of per-cell wire costs f(r-1) - f(r)
+ * along each run and a per-cell via cost (LPATTERN::get_gpu_lpat_dist,
+ * l_pattern_files/lpattern.hpp:13-160, util.hpp:133-353), and the
+ * difference-array commit: -1.0 on every cell a wire of the batch covers
+ * ([min, max) of its run), -0.5 per via half charge (MST_COMMIT::
+ * gpu_commit_edges, mst_files/mst_commit.hpp:17-364).  The state is fp32
+ * remaining capacity (gr_get_remaining_f32; gr_get_demand and gr_evaluate
+ * give GR_ESTATE).  The reference's via-cost cube is written by racing
+ * threads at transposed slots; here the via cost of a layer at a cell is the
+ * increase of f over exactly the capacity the commit's via charge takes. */
+#define GR_COMMIT_PREFIX 2
 
 /* tree modes */
 #define GR_TREE_ROOT_PAIR 0  /* record the component-root pair (gmst_util.hpp:245, 312, 322), default */
@@ -199,8 +211,14 @@ int gr_reset_demand(gr_ctx* ctx);
 /* Integer demand per cell (CSIZE int32), the parity state. */
 int gr_get_demand(gr_ctx* ctx, int32_t* out);
 
-/* Materialised Fenwick prefix Q per cell (CSIZE fp32), for parity tests. */
+/* Materialised Fenwick prefix Q per cell (CSIZE fp32), for parity tests; in
+ * the non-BIT mode the exclusive run prefix P of the wire costs. */
 int gr_get_prefix(gr_ctx* ctx, float* out);
+
+/* Per-cell via cost term (CSIZE fp32): Q(pos) - Q(pos-1) of the Fenwick
+ * prefix in the BIT modes; the via cost of the layer at the cell in the
+ * non-BIT mode.  For parity tests. */
+int gr_get_via_terms(gr_ctx* ctx, float* out);
 
 /* Number of pin pairs in the last routed batch. */
 int64_t gr_last_pairs(const gr_ctx* ctx);

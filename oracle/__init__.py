@@ -42,6 +42,10 @@ _SIGS = {
     "orc_set_tree_mode": (None, [_P, ctypes.c_int]),
     "orc_route_batch": (_I64, [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]),
     "orc_last_pairs": (_I64, [_P]),
+    "orc_get_vterm": (ctypes.c_int, [_P, _P]),
+    "orc_inc_cost": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_edge_commit_prefix": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int, _P, ctypes.c_int, _P, _P, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
     "orc_partition": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int,
                                      _P, _P, ctypes.c_int, _P]),
     "orc_get_pair_costs": (ctypes.c_int, [_P, _P, _P]),
@@ -105,6 +109,11 @@ def partition(X, Y, pin_pfx, pin_enc, parts=4, tile=32, trace_batch=-1):
     assert nb >= 0
     res = (ofst[:nb + 1].copy(), ids[:n].copy())
     return res + (trace[:n].copy(),) if trace_batch >= 0 else res
+
+
+def inc_cost(r, d):
+    """f(r - d) - f(r) in fp32 with f(r) = e^(-r/2) (the non-BIT mode's cell costs)."""
+    return np.float32(lib().orc_inc_cost(float(np.float32(r)), float(np.float32(d))))
 
 
 def cw_exp(x):
@@ -225,6 +234,23 @@ class Oracle:
         c = ctypes.c_float()
         p = lib().orc_best_pattern(self.h, int(src), int(dst), ctypes.byref(c))
         return p, c.value
+
+    def via_terms(self):
+        """Via cost per cell (non-BIT mode)."""
+        out = np.empty(self.X * self.Y * self.Z, dtype=np.float32)
+        assert lib().orc_get_vterm(self.h, _ptr(out)) == 0
+        return out
+
+    def edge_commit_prefix(self, src, dst, p):
+        """Non-BIT commit of one edge: (wire cells with repeats, [(cell, halves)])."""
+        w = np.empty(1 << 16, np.int64)
+        hc = np.empty(1 << 12, np.int64)
+        hn = np.empty(1 << 12, np.int32)
+        nh = ctypes.c_int()
+        nw = lib().orc_edge_commit_prefix(self.h, int(src), int(dst), int(p), _ptr(w), len(w), _ptr(hc), _ptr(hn),
+                                          len(hc), ctypes.byref(nh))
+        assert nw <= len(w) and nh.value <= len(hc)
+        return w[:nw].copy(), list(zip(hc[:nh.value].tolist(), hn[:nh.value].tolist()))
 
     def edge_cells(self, src, dst, p):
         buf = np.empty(4096, dtype=np.int64)

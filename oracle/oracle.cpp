@@ -103,6 +103,18 @@ float cost_f32(int model, float r) {
   return v;
 }
 
+// Increment of the base cost f(r) = e^(-r/2) when r drops by d:
+// f(r - d) - f(r), r - d in fp32 (get_gpu_of_cost_inc, util.hpp:127-129);
+// each f rounded to fp32 from the double cw_exp; saturated at 1e30 (an
+// infinite or NaN difference also saturates).  d = 1 is the wire cost of the
+// non-BIT mode, d = 0.5 half a via (util.hpp:149-193).
+float inc_cost(float r, float d) {
+  const float rd = r - d;
+  float v = (float)cw_exp(-0.5 * (double)rd) - (float)cw_exp(-0.5 * (double)r);
+  if (!(v < kSat)) v = kSat;
+  return v;
+}
+
 inline uint32_t ord_key(float f) {
   uint32_t u;
   f = f + 0.0f;  // -0 -> +0 so that -0 and +0 compare equal as in float ==
@@ -117,6 +129,11 @@ struct Ctx {
   std::vector<int32_t> cap0, rem, stamp;
   bool fmode = false;       // fp32 capacities: remf is the state, rem counts -demand
   std::vector<float> remf;
+  // non-BIT mode (commit_mode 2): remf is the state; pfx holds the wire
+  // costs' exclusive run prefix P, vterm the via cost per cell
+  bool prefix = false;
+  std::vector<float> vterm;
+  int chunk = 1;  // scan chunk: ceil(max(X, Y) / 128)
   std::vector<float> pfx;  // materialised Q(x) per cell, WIRE_ENC order
   float lut[kLut];
   int32_t bstamp = 0;
@@ -154,8 +171,50 @@ struct Ctx {
     return fmode ? cost_f32(model, remf[cell]) : cost_of_rem(rem[cell]);
   }
 
+  // Non-BIT cost state of one run (z, line) from its remaining capacities,
+  // rebuilt before every batch in the reference (test_lpat.hpp:281-289):
+  //   w(c) = inc_cost(r_c, 1)                        get_gpu_cost_cube, util.hpp:145-146
+  //   P    = exclusive prefix of w along the run, in the chunked order of
+  //          d_get_epfx_cost_cube_inplace (util.hpp:277-353, launched with
+  //          128 scanning threads over max(X, Y) cells, test_lpat.hpp:270-273):
+  //          chunk t covers [tT, tT + T), T = ceil(max(X, Y) / 128); its local
+  //          scan starts from the raw cost of the cell before the chunk, and
+  //          P[i] = P[tT - 1] + local(i) for t >= 1 (local(i) alone for t = 0)
+  //   V(c) = via cost of the layer at c.  The reference writes the two halves
+  //          of a via's charge to VIA_ENC slots vtid - 1 and vtid
+  //          (util.hpp:147-193); VIA_ENC puts z fastest (:41-42), so vtid - 1
+  //          is another layer's slot at a transposed cell, written by racing
+  //          threads.  Contract: V(c) is the increase in cost of exactly the
+  //          capacity the commit's via charge takes (mst_commit.hpp:17-61):
+  //          position 0 of the run: inc_cost(r_c, 1); last position:
+  //          inc_cost(r_(c-1), 1); otherwise inc_cost(r_(c-1), 0.5) +
+  //          inc_cost(r_c, 0.5) in fp32, left half first.
+  void refresh_run_prefix(int z, int line) {
+    const int L = run_len(z);
+    const int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
+    std::vector<float> w(L);
+    for (int i = 0; i < L; ++i) w[i] = inc_cost(remf[base + i], 1.0f);
+    for (int i = 0; i < L; ++i) {
+      float v;
+      if (i == 0) v = inc_cost(remf[base], 1.0f);
+      else if (i == L - 1) v = inc_cost(remf[base + i - 1], 1.0f);
+      else v = inc_cost(remf[base + i - 1], 0.5f) + inc_cost(remf[base + i], 0.5f);
+      vterm[base + i] = v;
+    }
+    const int T = chunk;
+    for (int t = 0; t * T < L; ++t) {
+      float local = t ? w[t * T - 1] : 0.0f;
+      const float carry = t ? pfx[base + t * T - 1] : 0.0f;
+      for (int i = t * T; i < std::min(L, (t + 1) * T); ++i) {
+        pfx[base + i] = t ? carry + local : local;
+        local += w[i];
+      }
+    }
+  }
+
   // Rebuild the materialised prefix of one run (z, line) from integer state.
   void refresh_run(int z, int line) {
+    if (prefix) { refresh_run_prefix(z, line); return; }
     int L = run_len(z);
     int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
     std::vector<std::vector<float>> lv;
@@ -230,16 +289,111 @@ struct Ctx {
     valid = true;
     return cost;
   }
+  // Non-BIT pattern cost, LPATTERN::get_lpat_dist (lpattern.hpp:98-160; the
+  // GPU kernel :13-97 walks the same route).  Wires add |P(end) - P(start)|
+  // (add_wires, lpattern_util.hpp:72-77): the cells [min, max) of the run.
+  // The first via stack adds V over layers [min, max) (add_vias :64-68 called
+  // with (zs, via1) or (via1, zs)), the corner and final stacks over the open
+  // interval (min, max) (called with zs + 1 / via2 + 1).  b' >= Z is invalid
+  // even for a straight pair, and the second leg is taken whenever the first
+  // wire does not end at the destination cell, so b matters for straight
+  // pairs too.  cost = wires + vias in fp32 (:155).
+  float pattern_cost_prefix(int64_t src, int64_t dst, int p, bool& valid) const {
+    int z1, x1, y1, z2, x2, y2;
+    dec(src, z1, x1, y1);
+    dec(dst, z2, x2, y2);
+    valid = false;
+    const int a = p / Z2;
+    int bp = p % Z2;
+    if (a == 0 || a >= Z) return 0.0f;
+    float wc = 0.0f, vc = 0.0f;
+    int zs = z1, xs = x1, ys = y1;
+    auto V = [&](int i, int x, int y) { return vterm[enc(i, x, y)]; };
+    auto Pat = [&](int z, int x, int y) { return pfx[enc(z, x, y)]; };
+    auto wire = [&](int z) {
+      if (z & 1) { wc += std::fabs(Pat(z, xs, y2) - Pat(z, xs, ys)); ys = y2; }
+      else       { wc += std::fabs(Pat(z, x2, ys) - Pat(z, xs, ys)); xs = x2; }
+    };
+    if (zs != a) {
+      for (int i = std::min(zs, a); i < std::max(zs, a); ++i) vc += V(i, xs, ys);
+      zs = a;
+    }
+    wire(zs);
+    bp = (zs & 1) ? 2 * bp : 2 * bp + 1;
+    if (bp >= Z) return 0.0f;
+    if (zs != z2 || xs != x2 || ys != y2) {
+      if (bp > 0) {
+        if (bp != zs) {
+          for (int i = std::min(zs, bp) + 1; i < std::max(zs, bp); ++i) vc += V(i, xs, ys);
+          zs = bp;
+        }
+        wire(zs);
+      }
+      if (zs != z2) {
+        for (int i = std::min(zs, z2) + 1; i < std::max(zs, z2); ++i) vc += V(i, xs, ys);
+        zs = z2;
+      }
+    }
+    if (xs != x2 || ys != y2 || zs != z2) return 0.0f;
+    valid = true;
+    return wc + vc;
+  }
+
   // Minimum over p of (cost, p), ascending p with strict <, lpattern.hpp:144-157.
   void best_pattern(int64_t src, int64_t dst, float& best, int& bp) const {
     bool have = false;
     best = 0.0f; bp = -1;
     for (int p = 0; p < Z * Z2; ++p) {
       bool v;
-      float c = pattern_cost(src, dst, p, v);
+      float c = prefix ? pattern_cost_prefix(src, dst, p, v) : pattern_cost(src, dst, p, v);
       if (!v) continue;
       if (!have || ord_key(c) < ord_key(best)) { best = c; bp = p; have = true; }
     }
+  }
+
+  // Non-BIT commit of one edge (MST_COMMIT, mst_commit.hpp:114-251): wire
+  // cells [min, max) of each wire (gpu_commit_wire :68-113, the difference
+  // array's +1 / -1), the first wire on a, the second on b' whenever b' > 0
+  // (gpu_add_wires :188-251); then via charges in half units along the route
+  // (gpu_add_vias :114-186): a stack from z1 to a at the source, a to b' at
+  // the position after the first wire when b' > 0, then to z2, each over
+  // layers [min, max) (gpu_commit_via :17-61): 2 halves to the cell at run
+  // position 0, 2 to the previous cell at the last position, else 1 to the
+  // previous cell and 1 to the cell.
+  void edge_cells_prefix(int64_t src, int64_t dst, int p, std::vector<int64_t>& wires,
+                         std::vector<std::pair<int64_t, int>>& halves) const {
+    int z1, x1, y1, z2, x2, y2;
+    dec(src, z1, x1, y1);
+    dec(dst, z2, x2, y2);
+    const int a = p / Z2, b = p % Z2;
+    int x = x1, y = y1;
+    auto wire = [&](int z) {
+      if (z & 1) { for (int t = std::min(y, y2); t < std::max(y, y2); ++t) wires.push_back(enc(z, x, t)); y = y2; }
+      else       { for (int t = std::min(x, x2); t < std::max(x, x2); ++t) wires.push_back(enc(z, t, y)); x = x2; }
+    };
+    wire(a);
+    const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+    if (bp > 0) wire(bp);
+    auto via = [&](int za, int zb) {
+      for (int i = std::min(za, zb); i < std::max(za, zb); ++i) {
+        const int pos = (i & 1) ? y : x, L = run_len(i);
+        const int64_t c = enc(i, x, y);
+        if (pos == 0) halves.emplace_back(c, 2);
+        else if (pos == L - 1) halves.emplace_back(c - 1, 2);
+        else { halves.emplace_back(c - 1, 1); halves.emplace_back(c, 1); }
+      }
+    };
+    x = x1; y = y1;
+    int z = z1;
+    via(z, a);
+    z = a;
+    if (z & 1) y = y2; else x = x2;
+    if (bp > 0) {
+      via(z, bp);
+      z = bp;
+      if (z & 1) y = y2; else x = x2;
+    }
+    if (z != z2) via(z, z2);
   }
 
   // mst_bit_commit.hpp:36-67,147-235: cells marked for one committed edge.
@@ -367,6 +521,13 @@ void* orc_create(int X, int Y, int Z, int cost_model, int commit_mode, const int
   c->stamp.assign(c->cs, 0);
   c->pfx.assign(c->cs, 0.0f);
   build_lut(cost_model, c->lut);
+  if (commit_mode == 2) {  // non-BIT mode: fp32 remaining capacity is the state
+    c->prefix = true;
+    c->fmode = true;
+    c->remf.assign(cap, cap + c->cs);
+    c->vterm.assign(c->cs, 0.0f);
+    c->chunk = (std::max(X, Y) + 127) / 128;
+  }
   c->refresh_all();
   return c;
 }
@@ -537,9 +698,32 @@ int orc_set_rem(void* h, const int32_t* rem) {
   return 0;
 }
 
+int orc_get_vterm(void* h, float* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c->prefix) return -1;
+  std::memcpy(out, c->vterm.data(), sizeof(float) * c->cs);
+  return 0;
+}
+
+float orc_inc_cost(float r, float d) { return inc_cost(r, d); }
+
+// Non-BIT commit of one edge: wire cells (with repeats) and (cell, halves).
+int orc_edge_commit_prefix(void* h, int32_t src, int32_t dst, int p, int64_t* wires, int wcap, int64_t* hcell,
+                           int32_t* hcount, int hcap, int* n_halves) {
+  Ctx* c = (Ctx*)h;
+  std::vector<int64_t> w;
+  std::vector<std::pair<int64_t, int>> hv;
+  c->edge_cells_prefix(src, dst, p, w, hv);
+  for (int i = 0; i < (int)std::min<size_t>(w.size(), (size_t)wcap); ++i) wires[i] = w[i];
+  for (int i = 0; i < (int)std::min<size_t>(hv.size(), (size_t)hcap); ++i) { hcell[i] = hv[i].first; hcount[i] = hv[i].second; }
+  *n_halves = (int)hv.size();
+  return (int)w.size();
+}
+
 float orc_pattern_cost(void* h, int32_t src, int32_t dst, int p, int* valid) {
   bool v;
-  float r = ((Ctx*)h)->pattern_cost(src, dst, p, v);
+  Ctx* c = (Ctx*)h;
+  float r = c->prefix ? c->pattern_cost_prefix(src, dst, p, v) : c->pattern_cost(src, dst, p, v);
   *valid = v;
   return r;
 }
@@ -624,6 +808,7 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
   c->last_cost.assign(pair_pfx[n], 0.0f);
   c->last_pat.assign(pair_pfx[n], -1);
   std::vector<std::vector<int64_t>> cells(n);
+  std::vector<std::vector<std::pair<int64_t, int>>> halves(n);  // non-BIT mode via charges
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < n; ++i) {
@@ -650,10 +835,50 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
       int hi = std::max(on1, on2), lo = std::min(on1, on2);
       int p = pp[pair_index(hi, lo)];
       out_lpat[base + t] = p;
-      c->edge_cells(pins[hi], pins[lo], p, cells[i]);
+      if (c->prefix) c->edge_cells_prefix(pins[hi], pins[lo], p, cells[i], halves[i]);
+      else c->edge_cells(pins[hi], pins[lo], p, cells[i]);
     }
     out_edges[base + d - 1] = -1;
     out_lpat[base + d - 1] = -1;
+  }
+  if (c->prefix) {
+    // mst_commit.hpp:342-364: every cell a wire of the batch covers loses 1.0
+    // once (the run scan of the difference array and update_cap_cube,
+    // :252-341); then the via charges (:114-186), summed per cell in half
+    // units and subtracted in one fp32 step (exact for half-integer states;
+    // the reference adds them with float atomics in any order).
+    c->bstamp++;
+    std::vector<char> dirty((size_t)c->Z * std::max(c->X, c->Y), 0);
+    auto mark_dirty = [&](int64_t cell) {
+      int z, x, y;
+      c->dec(cell, z, x, y);
+      dirty[(size_t)z * std::max(c->X, c->Y) + ((z & 1) ? x : y)] = 1;
+    };
+    int64_t touched = 0;
+    for (int i = 0; i < n; ++i)
+      for (int64_t cell : cells[i]) {
+        if (c->stamp[cell] == c->bstamp) continue;
+        c->stamp[cell] = c->bstamp;
+        c->remf[cell] -= 1.0f;
+        ++touched;
+        mark_dirty(cell);
+      }
+    std::vector<std::pair<int64_t, int>> all;
+    for (int i = 0; i < n; ++i) all.insert(all.end(), halves[i].begin(), halves[i].end());
+    std::sort(all.begin(), all.end());
+    for (size_t k = 0; k < all.size();) {
+      const int64_t cell = all[k].first;
+      int h = 0;
+      for (; k < all.size() && all[k].first == cell; ++k) h += all[k].second;
+      c->remf[cell] -= 0.5f * (float)h;
+      mark_dirty(cell);
+    }
+    for (int z = 0; z < c->Z; ++z) {
+      int lines = (z & 1) ? c->X : c->Y;
+      for (int l = 0; l < lines; ++l)
+        if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) c->refresh_run(z, l);
+    }
+    return touched;
   }
   // bit_util.hpp:75-90: each flagged cell loses one unit of capacity once.
   c->bstamp++;
