@@ -25,6 +25,13 @@
 
 namespace grd {
 
+// Phase markers of the workgroup tree (no-ops unless a profiling build
+// defines them).
+#ifndef GR_TREE_MARK
+#define GR_TREE_MARK(id)
+#define GR_TREE_MARK_INIT
+#endif
+
 // ---------------------------------------------------------------------------
 // Cost refresh
 // ---------------------------------------------------------------------------
@@ -453,8 +460,7 @@ __device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
 
 // Root-pair Boruvka (gmst_util.hpp:164-341) with the lowest-index tie contract
 // for one net on one workgroup (node r handled by thread r mod gsize):
-//   min phase    per component root, min (cost, o) over outgoing pairs; the
-//                pair costs are staged in LDS when they fit, else streamed
+//   min phase    per component root, min (cost, o) over outgoing pairs
 //   partner      root of the other endpoint of that pair
 //   accept       the reference processes candidates serially in ascending
 //                root order and drops one whose roots already joined.  With
@@ -467,30 +473,116 @@ __device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
 //                then flattened (root = the component's largest member).
 // Rounds repeat until deg-1 edges (the reference's feedback loop,
 // mst_util.hpp:298-314, ends in the same state).
-__device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int* root, unsigned long long* best,
-                            int* partner, int* par_a, int* par_b, int2* edges, float* ecost, int* wcnt, int* sh,
-                            float* scost, int scap) {
+//
+// The min phase reads pair keys while many components remain: staged in LDS
+// as ord_key values when they fit (scap), else streamed from global memory.
+// Once the C current components fit a triangular matrix in LDS (mcap 64-bit
+// entries, with room for the next round's matrix of at most C/2
+// components), the keys are read one last time to build M[a][b] = the
+// minimum (key << 32 | o) over the pairs joining components a and b, and
+// every later round works on that matrix: component a's minimum outgoing
+// pair is the minimum of its row and column, and after the round's merges
+// M' [a'][b'] is the minimum of the entries of the old components that
+// formed a' and b'.  A minimum over a union of pair sets is the minimum of
+// the minima, so each round selects exactly the pairs the key scan selects;
+// the matrix holds C(C-1)/2 entries instead of the n(n-1)/2 keys, and C at
+// least halves every round.
+//
+// LDS: node arrays root, partner (also the compact index of each root, cid),
+// par_a, par_b, ecost, best and edges (n each), croot_a / croot_b (the root of
+// each component), the matrix (mcap), the staged keys (scap).
+__device__ __forceinline__ uint32_t tri_index(int a, int b) {  // a > b
+  return (uint32_t)a * (uint32_t)(a - 1) / 2 + (uint32_t)b;
+}
+struct TreeLds {
+  int* root;
+  unsigned long long* best;
+  int* partner;
+  int* par_a;
+  int* par_b;
+  int2* edges;
+  float* ecost;
+  int* cid;
+  int* croot_a;
+  int* croot_b;
+  unsigned long long* mat;
+  int mcap;
+  int* wcnt;
+  int* sh;
+  uint32_t* skey;
+  int scap;
+};
+
+// Compact indices of the current roots in ascending node order: cid[r] for
+// every root r, croot[c] = the root of component c; returns the count.
+__device__ __forceinline__ int tree_ids(const TreeLds& L, int n, int tid, int gsize, int* croot) {
+  const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
+  const int iters = (n + gsize - 1) / gsize;
+  for (int it = 0; it < iters; ++it) {
+    const int i = it * gsize + tid;
+    const bool isr = i < n && L.root[i] == i;
+    const unsigned long long m = __ballot(isr);
+    if (lane == 0) L.wcnt[it * nwv + wv] = __popcll(m);
+  }
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+    const int i = it * gsize + tid;
+    const bool isr = i < n && L.root[i] == i;
+    const unsigned long long m = __ballot(isr);
+    if (isr) {
+      int off = __popcll(m & ((1ull << lane) - 1ull));
+      for (int q = 0; q < it * nwv + wv; ++q) off += L.wcnt[q];
+      L.cid[i] = off;
+      croot[off] = i;
+    }
+  }
+  int total = 0;
+  for (int q = 0; q < iters * nwv; ++q) total += L.wcnt[q];
+  __syncthreads();
+  return total;
+}
+
+template <bool MAT>
+__device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, const TreeLds& L) {
+  GR_TREE_MARK_INIT
+  GR_TREE_MARK(0);
+  int* root = L.root;
+  unsigned long long* best = L.best;
+  int* partner = L.partner;
+  int2* edges = L.edges;
+  float* ecost = L.ecost;
+  int* wcnt = L.wcnt;
+  int* sh = L.sh;
   const int64_t pb = A.pinofs[pos];
   const int n = A.deg[pos];
   const float* gcost = A.pair_cost + A.pair[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
   const uint32_t P = (uint32_t)((int64_t)n * (n - 1) / 2);
   const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
-  // every round reads all P costs: keep them in LDS when they fit, as the
+  // every key round reads all P costs: keep them in LDS when they fit, as the
   // orderable keys the minimum compares (ord_key), converted once here
   const float* cost = gcost;
-  const bool staged = P <= (uint32_t)scap;
-  uint32_t* skey = (uint32_t*)scost;
+  const bool staged = P <= (uint32_t)L.scap;
+  uint32_t* skey = L.skey;
   if (staged) {
     for (uint32_t o = tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
   }
   for (int i = tid; i < n; i += gsize) {
     root[i] = i;
     best[i] = ~0ull;
-    par_a[i] = i;
+    L.par_a[i] = i;
   }
   if (tid == 0) sh[0] = 0;
   __syncthreads();
+  GR_TREE_MARK(1);
+  // matrix rounds: C components, matrix in buffer mb (the other buffer gets
+  // the next round's), croot_cur / croot_nxt the components' roots
+  bool matrix = false;
+  int C = n;
+  unsigned long long* mcur = L.mat;
+  unsigned long long* mnxt = L.mat;
+  int* croot_cur = L.croot_a;
+  int* croot_nxt = L.croot_b;
   // Each round adds at least one edge (the globally smallest key is a mutual
   // pick), so n rounds always suffice; the bound only guards termination.
   // The DSU parents alternate between two arrays: the flatten of one round
@@ -499,9 +591,34 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
   for (int round = 0; round < n; ++round) {
     const int ne = sh[0];
     if (ne >= n - 1) break;
-    int* par = (round & 1) ? par_b : par_a;
-    int* par_next = (round & 1) ? par_a : par_b;
-    if (!staged) {
+    int* par = (round & 1) ? L.par_b : L.par_a;
+    int* par_next = (round & 1) ? L.par_a : L.par_b;
+    if (MAT && matrix) {
+      // component a: minimum over its row (b < a) and column (b > a); S
+      // adjacent threads per component, the result written by the first
+      const int S = (C * 8 <= gsize) ? 8 : (C * 4 <= gsize) ? 4 : (C * 2 <= gsize) ? 2 : 1;
+      const int per = gsize / S;
+      for (int b0 = 0; b0 < C; b0 += per) {
+        const int a = b0 + tid / S, part = tid & (S - 1);
+        unsigned long long m = ~0ull;
+        if (a < C) {
+          const uint32_t rb = tri_index(a, 0);
+          for (int b = part; b < a; b += S) {
+            const unsigned long long v = mcur[rb + b];
+            m = v < m ? v : m;
+          }
+          for (int b = a + 1 + ((part - (a + 1)) & (S - 1)); b < C; b += S) {
+            const unsigned long long v = mcur[tri_index(b, a)];
+            m = v < m ? v : m;
+          }
+        }
+        for (int sh2 = 1; sh2 < S; sh2 <<= 1) {
+          const unsigned long long t = shfl_xor_u64(m, sh2);
+          m = t < m ? t : m;
+        }
+        if (a < C && part == 0) best[croot_cur[a]] = m;
+      }
+    } else if (!staged) {
       // costs did not fit in LDS: each thread streams a contiguous chunk of
       // pairs from global memory.  The row root's running minimum and that of
       // the current run of equal column roots stay in registers and go to LDS
@@ -585,6 +702,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
       }
     }
     __syncthreads();
+    GR_TREE_MARK(2);
     for (int r = tid; r < n; r += gsize) {
       int pr = -1;
       if (root[r] == r && best[r] != ~0ull) {
@@ -595,6 +713,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
       partner[r] = pr;
     }
     __syncthreads();
+    GR_TREE_MARK(3);
     // accept, ordered slots, union
     const int iters = (n + gsize - 1) / gsize;
     for (int it = 0; it < iters; ++it) {
@@ -618,11 +737,10 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      int tot = ne;
-      for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
-      sh[0] = tot;
-    }
+    GR_TREE_MARK(4);
+    int tot = ne;
+    for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
+    if (tid == 0) sh[0] = tot;
     // flatten: every node's root becomes its component's largest member
     for (int i = tid; i < n; i += gsize) {
       root[i] = dsu_find(par, root[i]);
@@ -630,6 +748,66 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
       par_next[i] = i;
     }
     __syncthreads();
+    GR_TREE_MARK(5);
+    if (!MAT) continue;
+    const int Cn = n - tot;  // components after this round's unions
+    if (Cn <= 1) continue;
+    const uint32_t Tn = tri_index(Cn, 0), Th = tri_index(Cn / 2, 0);
+    if (matrix) {
+      // M' from M: the minimum over the old components forming each new pair
+      const int C2 = tree_ids(L, n, tid, gsize, croot_nxt);
+      (void)C2;
+      unsigned long long* mout = (mcur == L.mat) ? L.mat + L.mcap - Tn : L.mat;
+      for (uint32_t e = tid; e < Tn; e += gsize) mout[e] = ~0ull;
+      __syncthreads();
+      for (int a = 1 + wv; a < C; a += nwv) {
+        const int na = L.cid[root[croot_cur[a]]];
+        const uint32_t rb = tri_index(a, 0);
+        for (int b = lane; b < a; b += 64) {
+          const int nb = L.cid[root[croot_cur[b]]];
+          if (na != nb) {
+            const unsigned long long v = mcur[rb + b];
+            atomicMin(&mout[na > nb ? tri_index(na, nb) : tri_index(nb, na)], v);
+          }
+        }
+      }
+      __syncthreads();
+      GR_TREE_MARK(8);
+      mcur = mout;
+      int* t = croot_cur;
+      croot_cur = croot_nxt;
+      croot_nxt = t;
+      C = Cn;
+    } else if ((uint64_t)Tn + Th <= (uint64_t)L.mcap) {
+      // switch to matrix rounds: one last pass over the keys builds M
+      C = tree_ids(L, n, tid, gsize, croot_cur);
+      mcur = L.mat;
+      for (uint32_t e = tid; e < Tn; e += gsize) mcur[e] = ~0ull;
+      __syncthreads();
+      const uint32_t chunk = (P + gsize - 1) / gsize;
+      uint32_t o = (uint32_t)tid * chunk;
+      const uint32_t oe = o + chunk < P ? o + chunk : P;
+      if (o < oe) {
+        int j, k;
+        pair_decode(o, j, k);
+        int cj = L.cid[root[j]];
+        for (; o < oe; ++o) {
+          const int ck = L.cid[root[k]];
+          if (ck != cj) {
+            const uint32_t kv = staged ? skey[o] : ord_key(cost[o]);
+            atomicMin(&mcur[cj > ck ? tri_index(cj, ck) : tri_index(ck, cj)], ((unsigned long long)kv << 32) | o);
+          }
+          if (++k == j) {
+            ++j;
+            k = 0;
+            if (j < n) cj = L.cid[root[j]];
+          }
+        }
+      }
+      __syncthreads();
+      GR_TREE_MARK(7);
+      matrix = true;
+    }
   }
   const int64_t sb = A.slot[pos];
   const int32_t* npins = A.pins + pb;
@@ -641,6 +819,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, int*
     A.out_lpat[sb + n - 1] = -1;
     A.out_cost[A.cost_base + pos] = nc;
   }
+  GR_TREE_MARK(6);
 }
 
 // The same root-pair Boruvka for nets of at most G pins (G in {8,16,32,64}):
@@ -793,50 +972,75 @@ __global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C)
   small_block(A, C, blockIdx.x, 4, lds);
 }
 
+// LDS carve-up of a workgroup tree for nets of up to cap pins, mcap matrix
+// entries and scap staged keys (the host sizes the launch with
+// tree_lds_bytes).  The compact ids of the roots share the partner array:
+// they are written after the flatten and read by the matrix merge or build
+// of the same round, and partner is written again only in the next round.
+// The component root lists hold the most components a matrix of mcap
+// entries can describe.
+__host__ __device__ inline int tree_ccap(int cap, int mcap) {
+  int c = 1;
+  while ((int64_t)c * (c - 1) / 2 <= mcap && c <= cap) ++c;
+  return c;
+}
+__host__ __device__ inline size_t tree_lds_bytes(int cap, int mcap, int scap) {
+  return (size_t)cap * 16 + (size_t)mcap * 8 + (size_t)cap * 20 + (size_t)tree_ccap(cap, mcap) * 8 +
+         (size_t)scap * 4;
+}
+__device__ inline TreeLds tree_lds(unsigned long long* shl, int cap, int mcap, int scap, int* wcnt, int* sh) {
+  TreeLds L;
+  const int ccap = tree_ccap(cap, mcap);
+  L.best = shl;
+  L.edges = (int2*)(L.best + cap);
+  L.mat = (unsigned long long*)(L.edges + cap);
+  L.mcap = mcap;
+  L.root = (int*)(L.mat + mcap);
+  L.partner = L.root + cap;
+  L.par_a = L.partner + cap;
+  L.par_b = L.par_a + cap;
+  L.ecost = (float*)(L.par_b + cap);
+  L.cid = L.partner;
+  L.croot_a = (int*)(L.ecost + cap);
+  L.croot_b = L.croot_a + ccap;
+  L.skey = (uint32_t*)(L.croot_b + ccap);
+  L.scap = scap;
+  L.wcnt = wcnt;
+  L.sh = sh;
+  return L;
+}
+
 // One workgroup per net of more than kSmallDeg pins; LDS sized by the largest
-// such net of the batch (cap nodes).
-__global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int scap) {
+// such net of the batch (cap nodes).  MAT: matrix rounds (launches whose
+// largest net's keys do not fit in LDS); without them the code is leaner.
+template <bool MAT>
+__global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int mcap, int scap) {
   TreeArgs A = A0;
   A.bstamp = *A0.ctr;
   extern __shared__ unsigned long long shl[];
-  unsigned long long* best = shl;
-  int2* edges = (int2*)(best + cap);
-  int* root = (int*)(edges + cap);
-  int* partner = root + cap;
-  int* link = partner + cap;
-  int* link2 = link + cap;
-  float* ecost = (float*)(link2 + cap);
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
-  float* scost = ecost + cap;  // scap floats
-  tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, link2, edges, ecost, wcnt,
-              s_sh, scost, scap);
+  const TreeLds L = tree_lds(shl, cap, mcap, scap, wcnt, s_sh);
+  tree_commit<MAT>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
 }
 
 // Small and large nets of one batch in one launch (no second stream, so no
 // fork and join between the costing and the refresh): workgroups
 // [0, n_large) take one large net each with all blockDim threads, the others
 // run small-net work on their first W wavefronts (the rest exit at once).
-// The dynamic LDS holds either the large tree (cap nodes, scap keys) or W
-// group areas; the launch takes the larger of the two.
-__global__ __launch_bounds__(1024) void k_tree_mixed(TreeArgs A0, SmallClasses C, int n_large, int cap, int scap,
-                                                     int W) {
+// The dynamic LDS holds either the large tree or W group areas; the launch
+// takes the larger of the two.
+template <bool MAT>
+__global__ __launch_bounds__(1024) void k_tree_mixed(TreeArgs A0, SmallClasses C, int n_large, int cap, int mcap,
+                                                     int scap, int W) {
   TreeArgs A = A0;
   A.bstamp = *A0.ctr;
   extern __shared__ unsigned long long shl[];
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
   if ((int)blockIdx.x < n_large) {
-    unsigned long long* best = shl;
-    int2* edges = (int2*)(best + cap);
-    int* root = (int*)(edges + cap);
-    int* partner = root + cap;
-    int* link = partner + cap;
-    int* link2 = link + cap;
-    float* ecost = (float*)(link2 + cap);
-    float* scost = ecost + cap;
-    tree_commit(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, root, best, partner, link, link2, edges, ecost, wcnt,
-                s_sh, scost, scap);
+    const TreeLds L = tree_lds(shl, cap, mcap, scap, wcnt, s_sh);
+    tree_commit<MAT>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
   } else {
     if ((int)(threadIdx.x >> 6) >= W) return;
     small_block(A, C, (int)blockIdx.x - n_large, W, (GroupLds*)shl);

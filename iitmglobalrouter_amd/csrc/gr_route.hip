@@ -325,9 +325,24 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
   C.block0[5] = blocks;
   const int n_large = m.cls_n[kClsLarge];
   const int cap = m.max_large;
-  const int64_t room = ((int64_t)c->lds_max - 2048 - (int64_t)cap * 36) / 4;
-  const int scap = (int)std::max<int64_t>(0, std::min<int64_t>(room, m.max_large_P));
-  const size_t large_lds = (size_t)cap * 36 + (size_t)scap * 4;
+  // LDS of the workgroup trees: node arrays, then the pair keys when the
+  // largest net's fit; otherwise keys are streamed from global memory and
+  // the rest of the LDS holds the component matrix (the matrix rounds start
+  // once a round leaves few enough components)
+  int scap = 0, mcap = 0;
+  bool mat = false;
+  if (n_large) {
+    const int64_t budget = (int64_t)c->lds_max - 2048;
+    if ((int64_t)tree_lds_bytes(cap, 0, (int)std::min<int64_t>(m.max_large_P, 1 << 30)) <= budget) {
+      scap = (int)m.max_large_P;
+    } else {
+      mat = true;
+      int64_t mc = (budget - (int64_t)tree_lds_bytes(cap, 0, 0)) / 8;
+      while (mc > 0 && (int64_t)tree_lds_bytes(cap, (int)mc, 0) > budget) mc -= 16;
+      mcap = (int)std::max<int64_t>(0, mc);
+    }
+  }
+  const size_t large_lds = n_large ? tree_lds_bytes(cap, mcap, scap) : 0;
   TreeArgs L = A;
   L.list = s.d_list + m.cls_ofs[kClsLarge];
   L.n_list = n_large;
@@ -350,8 +365,8 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
         mb += (Cm.n[k] + per_block - 1) / per_block;
       }
       Cm.block0[5] = mb;
-      hipLaunchKernelGGL(k_tree_mixed, dim3(n_large + mb), dim3(c->tune.large_threads), lds, c->stream, L, Cm,
-                         n_large, cap, scap, W);
+      hipLaunchKernelGGL(mat ? k_tree_mixed<true> : k_tree_mixed<false>, dim3(n_large + mb),
+                         dim3(c->tune.large_threads), lds, c->stream, L, Cm, n_large, cap, mcap, scap, W);
       return 0;
     }
   }
@@ -363,7 +378,9 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
     lst = c->stream2;
   }
   if (blocks) hipLaunchKernelGGL(k_tree_small, dim3(blocks), dim3(256), 0, c->stream, A, C);
-  if (n_large) hipLaunchKernelGGL(k_tree_large, dim3(n_large), dim3(c->tune.large_threads), large_lds, lst, L, cap, scap);
+  if (n_large)
+    hipLaunchKernelGGL(mat ? k_tree_large<true> : k_tree_large<false>, dim3(n_large), dim3(c->tune.large_threads),
+                       large_lds, lst, L, cap, mcap, scap);
   if (fork) {
     HIPC(c, hipEventRecord(c->ev_join, c->stream2));
     HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -554,9 +571,13 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
          hipMemcpy(c->d_remf, capf, sizeof(float) * cs, hipMemcpyHostToDevice) == hipSuccess;
   }
   if (!ok ||
-      hipFuncSetAttribute((const void*)k_tree_large, hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute((const void*)k_tree_large<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
-      hipFuncSetAttribute((const void*)k_tree_mixed, hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute((const void*)k_tree_large<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)c->lds_max - 2048) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tree_mixed<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)c->lds_max - 2048) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tree_mixed<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
