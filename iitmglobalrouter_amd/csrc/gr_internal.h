@@ -35,6 +35,7 @@ struct BatchMeta {
   int64_t slot0, nslots;  // output slots of the batch
   int32_t max_large;      // most pins of a large net in the batch
   int64_t max_large_P;    // most pin pairs of a large net in the batch
+  int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
 };
 
 struct Sched {
@@ -98,6 +99,7 @@ struct gr_ctx {
   gr_tuning tune = {1024, 64, 3, 2048, 0, 0};
   int32_t n_nets = 0;
   std::vector<int64_t> h_pin_pfx;
+  std::vector<int32_t> h_net_runs;  // per net: bound on the runs its routes can touch
   int64_t* d_pin_pfx = nullptr;
   int32_t* d_pins = nullptr;
   grh::Sched sched, single;

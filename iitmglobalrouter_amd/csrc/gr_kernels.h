@@ -277,26 +277,27 @@ __device__ __forceinline__ int2 recorded_edge(const TreeArgs& A, const int* root
 // Commit marks are idempotent plain stores: the cell's mark byte and its
 // run's stamp.  The demand update itself happens once per marked cell in
 // k_refresh, which also clears the mark.
-__device__ __forceinline__ void mark_run(const TreeArgs& A, int z, int line) {
+__device__ __forceinline__ void mark_run(const TreeArgs& A, int z, int line, int pos) {
+  (void)pos;
   A.run_stamp[z * A.M + line] = A.bstamp;
 }
 __device__ __forceinline__ void mark_any(const TreeArgs& A, uint32_t cell) {
   A.mark[cell] = 1;
   int z, x, y;
   dec(A.g, cell, z, x, y);
-  mark_run(A, z, (z & 1) ? x : y);
+  mark_run(A, z, (z & 1) ? x : y, (z & 1) ? y : x);
 }
 __device__ __forceinline__ void mark_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
   const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
   const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
   for (int t = lo; t <= hi; ++t) A.mark[base + t] = 1;
-  mark_run(A, z, line);
+  mark_run(A, z, line, lo);
 }
 __device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int x, int y) {
   const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
   for (int l = lo + 1; l < hi; ++l) {
     A.mark[enc(A.g, l, x, y)] = 1;
-    mark_run(A, l, (l & 1) ? x : y);
+    mark_run(A, l, (l & 1) ? x : y, (l & 1) ? y : x);
   }
 }
 
@@ -321,7 +322,7 @@ __device__ __forceinline__ void prefix_wire(const TreeArgs& A, int z, int line, 
   const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
   const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
   for (int t = lo; t < hi; ++t) A.mark[base + t] = 1;
-  mark_run(A, z, line);
+  A.run_stamp[z * A.M + line] = A.bstamp;
 }
 __device__ __forceinline__ void prefix_via(const TreeArgs& A, int za, int zb, int x, int y) {
   const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
@@ -331,7 +332,7 @@ __device__ __forceinline__ void prefix_via(const TreeArgs& A, int za, int zb, in
     if (pos == 0) atomicAdd(&A.vhalf[c], 2);
     else if (pos == L - 1) atomicAdd(&A.vhalf[c - 1], 2);
     else { atomicAdd(&A.vhalf[c - 1], 1); atomicAdd(&A.vhalf[c], 1); }
-    mark_run(A, i, (i & 1) ? x : y);
+    A.run_stamp[i * A.M + ((i & 1) ? x : y)] = A.bstamp;
   }
 }
 __device__ void commit_edge_prefix(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
@@ -405,6 +406,23 @@ __device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p
     else       { mark_wire(A, z, y, min(x, x2), max(x, x2)); x = x2; }
   }
   if (z != z2) mark_via(A, z, z2, x, y);
+}
+
+// Net cost: the n-1 accepted minimum costs added in slot order (the
+// reference's serial net_dist += cost), the LDS loads issued in groups of 8
+// ahead of the dependent adds.
+__device__ __forceinline__ float sum_edge_costs(const float* ecost, int cnt) {
+  float nc = 0.0f;
+  int e = 0;
+  for (; e + 8 <= cnt; e += 8) {
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = ecost[e + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) nc += v[q];
+  }
+  for (; e < cnt; ++e) nc += ecost[e];
+  return nc;
 }
 
 __device__ __forceinline__ void emit_edge(const TreeArgs& A, int pos, int64_t sb, int e, int on1, int on2,
@@ -812,12 +830,11 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
   const int64_t sb = A.slot[pos];
   const int32_t* npins = A.pins + pb;
   for (int e = tid; e < n - 1; e += gsize) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
-  if (tid == 0) {
-    float nc = 0.0f;
-    for (int e = 0; e < n - 1; ++e) nc += ecost[e];
+  // the net cost on a thread without an edge when there is one
+  if (tid == (n - 1) % gsize) {
     A.out_edges[sb + n - 1] = -1;
     A.out_lpat[sb + n - 1] = -1;
-    A.out_cost[A.cost_base + pos] = nc;
+    A.out_cost[A.cost_base + pos] = sum_edge_costs(ecost, n - 1);
   }
   GR_TREE_MARK(6);
 }
@@ -921,12 +938,10 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
   const int16_t* pat = A.pair_pat + A.pair[pos];
   const int32_t* npins = A.pins + pb;
   for (int e = tid; e < n - 1; e += G) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
-  if (tid == 0) {
-    float nc = 0.0f;
-    for (int e = 0; e < n - 1; ++e) nc += ecost[e];
+  if (tid == n - 1) {  // a lane without an edge
     A.out_edges[sb + n - 1] = -1;
     A.out_lpat[sb + n - 1] = -1;
-    A.out_cost[A.cost_base + pos] = nc;
+    A.out_cost[A.cost_base + pos] = sum_edge_costs(ecost, n - 1);
   }
 }
 

@@ -108,18 +108,26 @@ struct KTimer {
 
 size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
 size_t refresh_prefix_lds(const gr_ctx* c) { return (3 * (size_t)c->M + 128) * sizeof(float); }
-int refresh_grid(const gr_ctx* c) { return std::max(1, std::min(c->g.Z * c->M, c->tune.refresh_blocks)); }
+// Refresh grid: the tuned size, or fewer workgroups when the batch can touch
+// fewer runs (runs < 0: every run, as at set-up and reset).  Workgroups scan
+// the runs with a stride, so any size visits every stamped run.
+int refresh_grid(const gr_ctx* c, int runs) {
+  int g = std::min(c->g.Z * c->M, c->tune.refresh_blocks);
+  if (runs >= 0) g = std::min(g, std::max(runs, 64));
+  return std::max(1, g);
+}
 
-void launch_refresh(gr_ctx* c, int all) {
+void launch_refresh(gr_ctx* c, int all, int runs = -1) {
   if (c->prefix)
-    hipLaunchKernelGGL(k_refresh_prefix, dim3(refresh_grid(c)), dim3(256), refresh_prefix_lds(c), c->stream, c->g, c->M,
-                       c->d_remf, c->d_mark, c->d_vhalf, c->d_pfx, c->d_vterm, c->d_run_stamp, c->d_ctr, all);
+    hipLaunchKernelGGL(k_refresh_prefix, dim3(refresh_grid(c, runs)), dim3(256), refresh_prefix_lds(c), c->stream,
+                       c->g, c->M, c->d_remf, c->d_mark, c->d_vhalf, c->d_pfx, c->d_vterm, c->d_run_stamp, c->d_ctr,
+                       all);
   else if (c->d_remf)
-    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
+    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
                        c->d_vterm, (int16_t*)nullptr);
   else
-    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
+    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
                        c->d_vterm, c->remz_live ? c->d_remz : (int16_t*)nullptr);
 }
@@ -150,7 +158,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     m.begin = batch_ofst[b];
     m.n = batch_ofst[b + 1] - batch_ofst[b];
     if (m.n < 0) return fail(c, GR_EINVAL, "batch offsets must be non-decreasing");
-    int64_t P = 0;
+    int64_t P = 0, runs = 0;
     m.max_large = 0;
     m.max_large_P = 0;
     m.slot0 = slots;
@@ -166,7 +174,9 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
       h.degv[m.begin + i] = (int32_t)d;
       slots += d;
       P += d * (d - 1) / 2;
+      runs += c->h_net_runs[id];
     }
+    m.runs = (int32_t)std::min<int64_t>(runs, (int64_t)c->g.Z * c->M);
     m.nslots = slots - m.slot0;
     for (int cls = 0; cls < kNumCls; ++cls) {
       m.cls_ofs[cls] = lo;
@@ -457,7 +467,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   }
   {
     KTimer t(c, GR_K_REFRESH);
-    launch_refresh(c, 0);
+    launch_refresh(c, 0, m.runs);
   }
   HIPC(c, hipGetLastError());
   c->last_P = m.P;
@@ -652,6 +662,25 @@ int gr_set_nets(gr_ctx* c, int32_t n_nets, const int64_t* pin_pfx, const int32_t
   HIPC(c, hipMemcpy(c->d_pin_pfx, pin_pfx, sizeof(int64_t) * (n_nets + 1), hipMemcpyHostToDevice));
   if (np) HIPC(c, hipMemcpy(c->d_pins, pin_enc, sizeof(int32_t) * np, hipMemcpyHostToDevice));
   c->h_pin_pfx.assign(pin_pfx, pin_pfx + n_nets + 1);
+  // runs a net's routes can touch: the rows of its box on even layers and
+  // the columns on odd ones, each with one more for the quirk cells that fall
+  // just before a first wire's low end (their enc arithmetic can reach the
+  // previous run)
+  c->h_net_runs.assign(n_nets, 0);
+  for (int32_t i = 0; i < n_nets; ++i) {
+    int x0 = c->g.X, x1 = -1, y0 = c->g.Y, y1 = -1;
+    for (int64_t p = pin_pfx[i]; p < pin_pfx[i + 1]; ++p) {
+      const uint32_t v = (uint32_t)pin_enc[p];
+      const int z = (int)(v / c->g.XY);
+      const uint32_t r = v - (uint32_t)z * c->g.XY;
+      const int x = (z & 1) ? (int)(r / c->g.Y) : (int)(r % c->g.X);
+      const int y = (z & 1) ? (int)(r % c->g.Y) : (int)(r / c->g.X);
+      x0 = std::min(x0, x); x1 = std::max(x1, x);
+      y0 = std::min(y0, y); y1 = std::max(y1, y);
+    }
+    const int64_t even = (c->g.Z + 1) / 2, odd = c->g.Z / 2;
+    c->h_net_runs[i] = (int32_t)std::min<int64_t>(even * (y1 - y0 + 2) + odd * (x1 - x0 + 2), (int64_t)c->g.Z * c->M);
+  }
   c->n_nets = n_nets;
   free_sched(c->sched);
   return 0;
