@@ -217,13 +217,18 @@ struct Ctx {
     if (prefix) { refresh_run_prefix(z, line); return; }
     int L = run_len(z);
     int64_t base = (int64_t)z * X * Y + (int64_t)line * L;
-    std::vector<std::vector<float>> lv;
-    lv.emplace_back(L);
-    for (int i = 0; i < L; ++i) lv[0][i] = cell_cost(base + i);
-    for (int k = 1; (L >> k) > 0; ++k) {
-      int n = L >> k;
-      lv.emplace_back(n);
-      for (int j = 0; j < n; ++j) lv[k][j] = lv[k - 1][2 * j] + lv[k - 1][2 * j + 1];
+    // level k holds the aligned block sums of 2^k cells, levels stored one
+    // after another in a per-thread buffer (level k starts at ofs[k])
+    static thread_local std::vector<float> lv;
+    int ofs[32];
+    size_t total = 0;
+    int nlev = 0;
+    for (int k = 0; (L >> k) > 0; ++k) { ofs[k] = (int)total; total += (size_t)(L >> k); nlev = k + 1; }
+    if (lv.size() < total) lv.resize(total);
+    for (int i = 0; i < L; ++i) lv[i] = cell_cost(base + i);
+    for (int k = 1; k < nlev; ++k) {
+      const int n = L >> k;
+      for (int j = 0; j < n; ++j) lv[ofs[k] + j] = lv[ofs[k - 1] + 2 * j] + lv[ofs[k - 1] + 2 * j + 1];
     }
     for (int x = 0; x < L; ++x) {
       // bit_util.hpp:19-35: res = 0; while (i > 0) { res += node[i]; i -= i & -i; }
@@ -231,17 +236,26 @@ struct Ctx {
       float res = 0.0f;
       while (i > 0) {
         int k = __builtin_ctz(i);
-        res += lv[k][(i >> k) - 1];
+        res += lv[ofs[k] + (i >> k) - 1];
         i &= i - 1;
       }
       pfx[base + x] = res;
     }
   }
+  // Rebuild the listed runs (run id z * max(X, Y) + line), in parallel.
+  void refresh_runs(const std::vector<int>& runs) {
+    const int M = std::max(X, Y);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (size_t i = 0; i < runs.size(); ++i) refresh_run(runs[i] / M, runs[i] % M);
+  }
   void refresh_all() {
+    std::vector<int> runs;
+    const int M = std::max(X, Y);
     for (int z = 0; z < Z; ++z) {
       int lines = (z & 1) ? X : Y;
-      for (int l = 0; l < lines; ++l) refresh_run(z, l);
+      for (int l = 0; l < lines; ++l) runs.push_back(z * M + l);
     }
+    refresh_runs(runs);
   }
 
   // bit_util.hpp:108-117: each via layer i in [min, max) adds Q(pos) - Q(pos-1).
@@ -873,11 +887,13 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
       c->remf[cell] -= 0.5f * (float)h;
       mark_dirty(cell);
     }
+    std::vector<int> runs;
     for (int z = 0; z < c->Z; ++z) {
       int lines = (z & 1) ? c->X : c->Y;
       for (int l = 0; l < lines; ++l)
-        if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) c->refresh_run(z, l);
+        if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) runs.push_back(z * std::max(c->X, c->Y) + l);
     }
+    c->refresh_runs(runs);
     return touched;
   }
   // bit_util.hpp:75-90: each flagged cell loses one unit of capacity once.
@@ -896,11 +912,13 @@ int64_t orc_route_batch(void* h, const int32_t* net_ids, int n, int64_t* out_edg
       int line = (z & 1) ? x : y;
       dirty[(size_t)z * std::max(c->X, c->Y) + line] = 1;
     }
+  std::vector<int> runs;
   for (int z = 0; z < c->Z; ++z) {
     int lines = (z & 1) ? c->X : c->Y;
     for (int l = 0; l < lines; ++l)
-      if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) c->refresh_run(z, l);
+      if (dirty[(size_t)z * std::max(c->X, c->Y) + l]) runs.push_back(z * std::max(c->X, c->Y) + l);
   }
+  c->refresh_runs(runs);
   return touched;
 }
 
