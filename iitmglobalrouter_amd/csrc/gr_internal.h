@@ -16,11 +16,15 @@ using grd::Geo;
 using grd::kLut;
 
 // Net size classes of the tree stage: 2 pins; up to 8, 16, 32, 64 pins
-// (one group of that many lanes per net); more (one workgroup per net).
-constexpr int kNumCls = 6;
+// (one group of that many lanes per net); more (one workgroup per net, node
+// arrays in LDS); more than kLargeDeg (one workgroup per net, node arrays in
+// global memory).
+constexpr int kNumCls = 7;
 constexpr int kClsLarge = 5;
-constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, 1 << 30};
+constexpr int kClsGiant = 6;
+constexpr int kClsMax[kNumCls] = {2, 8, 16, 32, 64, grd::kLargeDeg, 1 << 30};
 inline int net_class(int64_t d, int group_max) {
+  if (d > grd::kLargeDeg) return kClsGiant;
   if (d > group_max) return kClsLarge;
   int k = 0;
   while (d > kClsMax[k]) ++k;
@@ -35,6 +39,7 @@ struct BatchMeta {
   int64_t slot0, nslots;  // output slots of the batch
   int32_t max_large;      // most pins of a large net in the batch
   int64_t max_large_P;    // most pin pairs of a large net in the batch
+  int32_t max_giant;      // most pins of a net above kLargeDeg in the batch (0: none)
   int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
 };
 
@@ -51,7 +56,9 @@ struct Sched {
   int64_t* d_edges = nullptr;
   int32_t* d_lpat = nullptr;
   float* d_cost = nullptr;
-  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0;
+  uint8_t* d_giant = nullptr;  // node arrays of the nets above kLargeDeg pins: 44 B per slot of the batch
+  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0;
+  int64_t giant_bytes = 0;     // largest need over the batches with such nets
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;

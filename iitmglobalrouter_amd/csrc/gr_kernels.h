@@ -531,8 +531,20 @@ struct TreeLds {
   int scap;
 };
 
+// Workgroup barrier of the tree; with the node arrays in global memory (GN)
+// it is also an agent-scope release and acquire, so stores are in the L2
+// before the barrier and the L1 holds no stale line after it (the atomics of
+// the min phase and the DSU hooking act in the L2).
+template <bool GN>
+__device__ __forceinline__ void tree_sync() {
+  if (GN) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (GN) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // Compact indices of the current roots in ascending node order: cid[r] for
 // every root r, croot[c] = the root of component c; returns the count.
+template <bool GN>
 __device__ __forceinline__ int tree_ids(const TreeLds& L, int n, int tid, int gsize, int* croot) {
   const int lane = tid & 63, wv = tid >> 6, nwv = (gsize + 63) >> 6;
   const int iters = (n + gsize - 1) / gsize;
@@ -542,7 +554,7 @@ __device__ __forceinline__ int tree_ids(const TreeLds& L, int n, int tid, int gs
     const unsigned long long m = __ballot(isr);
     if (lane == 0) L.wcnt[it * nwv + wv] = __popcll(m);
   }
-  __syncthreads();
+  tree_sync<GN>();
   for (int it = 0; it < iters; ++it) {
     const int i = it * gsize + tid;
     const bool isr = i < n && L.root[i] == i;
@@ -556,11 +568,11 @@ __device__ __forceinline__ int tree_ids(const TreeLds& L, int n, int tid, int gs
   }
   int total = 0;
   for (int q = 0; q < iters * nwv; ++q) total += L.wcnt[q];
-  __syncthreads();
+  tree_sync<GN>();
   return total;
 }
 
-template <bool MAT>
+template <bool MAT, bool GN = false>
 __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, const TreeLds& L) {
   GR_TREE_MARK_INIT
   GR_TREE_MARK(0);
@@ -591,7 +603,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
     L.par_a[i] = i;
   }
   if (tid == 0) sh[0] = 0;
-  __syncthreads();
+  tree_sync<GN>();
   GR_TREE_MARK(1);
   // matrix rounds: C components, matrix in buffer mb (the other buffer gets
   // the next round's), croot_cur / croot_nxt the components' roots
@@ -719,7 +731,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
         if (i < n && part == 0 && m != ~0ull) atomicMin(&best[ri], m);
       }
     }
-    __syncthreads();
+    tree_sync<GN>();
     GR_TREE_MARK(2);
     for (int r = tid; r < n; r += gsize) {
       int pr = -1;
@@ -730,7 +742,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       }
       partner[r] = pr;
     }
-    __syncthreads();
+    tree_sync<GN>();
     GR_TREE_MARK(3);
     // accept, ordered slots, union
     const int iters = (n + gsize - 1) / gsize;
@@ -744,7 +756,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       }
       const unsigned long long m = __ballot(acc);
       if (lane == 0) wcnt[it * nwv + wv] = __popcll(m);
-      __syncthreads();
+      tree_sync<GN>();
       if (acc) {
         int off = ne;
         for (int q = 0; q < it * nwv + wv; ++q) off += wcnt[q];
@@ -754,7 +766,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
         dsu_union(par, r, pr, 4 * n);
       }
     }
-    __syncthreads();
+    tree_sync<GN>();
     GR_TREE_MARK(4);
     int tot = ne;
     for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
@@ -765,7 +777,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       best[i] = ~0ull;
       par_next[i] = i;
     }
-    __syncthreads();
+    tree_sync<GN>();
     GR_TREE_MARK(5);
     if (!MAT) continue;
     const int Cn = n - tot;  // components after this round's unions
@@ -773,11 +785,11 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
     const uint32_t Tn = tri_index(Cn, 0), Th = tri_index(Cn / 2, 0);
     if (matrix) {
       // M' from M: the minimum over the old components forming each new pair
-      const int C2 = tree_ids(L, n, tid, gsize, croot_nxt);
+      const int C2 = tree_ids<GN>(L, n, tid, gsize, croot_nxt);
       (void)C2;
       unsigned long long* mout = (mcur == L.mat) ? L.mat + L.mcap - Tn : L.mat;
       for (uint32_t e = tid; e < Tn; e += gsize) mout[e] = ~0ull;
-      __syncthreads();
+      tree_sync<GN>();
       for (int a = 1 + wv; a < C; a += nwv) {
         const int na = L.cid[root[croot_cur[a]]];
         const uint32_t rb = tri_index(a, 0);
@@ -789,7 +801,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
           }
         }
       }
-      __syncthreads();
+      tree_sync<GN>();
       GR_TREE_MARK(8);
       mcur = mout;
       int* t = croot_cur;
@@ -798,10 +810,10 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       C = Cn;
     } else if ((uint64_t)Tn + Th <= (uint64_t)L.mcap) {
       // switch to matrix rounds: one last pass over the keys builds M
-      C = tree_ids(L, n, tid, gsize, croot_cur);
+      C = tree_ids<GN>(L, n, tid, gsize, croot_cur);
       mcur = L.mat;
       for (uint32_t e = tid; e < Tn; e += gsize) mcur[e] = ~0ull;
-      __syncthreads();
+      tree_sync<GN>();
       const uint32_t chunk = (P + gsize - 1) / gsize;
       uint32_t o = (uint32_t)tid * chunk;
       const uint32_t oe = o + chunk < P ? o + chunk : P;
@@ -822,7 +834,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
           }
         }
       }
-      __syncthreads();
+      tree_sync<GN>();
       GR_TREE_MARK(7);
       matrix = true;
     }
@@ -1037,6 +1049,41 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int m
   __shared__ int s_sh[2];
   const TreeLds L = tree_lds(shl, cap, mcap, scap, wcnt, s_sh);
   tree_commit<MAT>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
+}
+
+// One workgroup per net of more than kLargeDeg pins: the node arrays live in
+// global memory (44 B per slot of the batch, at the net's slot offset), the
+// LDS holds the component matrix; keys are always streamed.  Global memory
+// written by the workgroup's own threads is seen by them after each barrier,
+// and the DSU hooking's compare-and-swap and volatile reads go to the L2.
+__global__ __launch_bounds__(1024) void k_tree_giant(TreeArgs A0, int mcap, uint8_t* scratch, int64_t nslots,
+                                                     int64_t slot0) {
+  TreeArgs A = A0;
+  A.bstamp = *A0.ctr;
+  extern __shared__ unsigned long long shl[];
+  __shared__ int wcnt[((kMaxDeg + 1023) / 1024) * 16];
+  __shared__ int s_sh[2];
+  const int pos = A.list[blockIdx.x];
+  const int64_t o = A.slot[pos] - slot0;
+  TreeLds L;
+  L.best = (unsigned long long*)scratch + o;
+  L.edges = (int2*)(scratch + nslots * 8) + o;
+  int* ib = (int*)(scratch + nslots * 16);
+  L.root = ib + o;
+  L.partner = ib + nslots + o;
+  L.par_a = ib + 2 * nslots + o;
+  L.par_b = ib + 3 * nslots + o;
+  L.ecost = (float*)(ib + 4 * nslots) + o;
+  L.cid = L.partner;
+  L.croot_a = ib + 5 * nslots + o;
+  L.croot_b = ib + 6 * nslots + o;
+  L.mat = shl;
+  L.mcap = mcap;
+  L.skey = nullptr;
+  L.scap = 0;
+  L.wcnt = wcnt;
+  L.sh = s_sh;
+  tree_commit<true, true>(A, pos, threadIdx.x, blockDim.x, L);
 }
 
 // Small and large nets of one batch in one launch (no second stream, so no

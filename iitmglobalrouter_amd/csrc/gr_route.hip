@@ -151,6 +151,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
   h.list.assign(npos, 0);
   h.chunk.clear();
   s.meta.assign(n_batches, BatchMeta{});
+  s.giant_bytes = 0;
   int64_t slots = 0, maxP = 0;
   int32_t lo = 0;
   for (int b = 0; b < n_batches; ++b) {
@@ -161,13 +162,14 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     int64_t P = 0, runs = 0;
     m.max_large = 0;
     m.max_large_P = 0;
+    m.max_giant = 0;
     m.slot0 = slots;
     for (int i = 0; i < m.n; ++i) {
       const int32_t id = net_ids[m.begin + i];
       if (id < 0 || id >= c->n_nets) return fail(c, GR_EINVAL, "net id out of range");
       const int64_t d = c->h_pin_pfx[id + 1] - c->h_pin_pfx[id];
-      if (d > kLargeDeg)
-        return fail(c, GR_ERANGE, "net has more pins than the device tree supports (" + std::to_string(kLargeDeg) + ")");
+      if (d > kMaxDeg)
+        return fail(c, GR_ERANGE, "net has more pins than the 15-bit pin index allows (" + std::to_string(kMaxDeg) + ")");
       h.slot[m.begin + i] = slots;
       h.pair[m.begin + i] = P;
       h.pinofs[m.begin + i] = c->h_pin_pfx[id];
@@ -190,6 +192,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
           m.max_large = std::max<int>(m.max_large, (int)d);
           m.max_large_P = std::max<int64_t>(m.max_large_P, d * (d - 1) / 2);
         }
+        if (k == kClsGiant) m.max_giant = std::max<int>(m.max_giant, (int)d);
       }
       m.cls_n[cls] = lo - m.cls_ofs[cls];
     }
@@ -202,6 +205,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     }
     m.P = P;
     maxP = std::max(maxP, P);
+    if (m.max_giant) s.giant_bytes = std::max<int64_t>(s.giant_bytes, m.nslots * 44);
   }
   s.n_pos = npos;
   s.slots = slots;
@@ -253,6 +257,10 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   if (h.chunk.size() > s.cap_chunk) {
     if ((rc = dalloc(c, &s.d_chunk, h.chunk.size()))) return rc;
     s.cap_chunk = h.chunk.size();
+  }
+  if ((size_t)s.giant_bytes > s.cap_giant) {
+    if ((rc = dalloc(c, &s.d_giant, (size_t)s.giant_bytes))) return rc;
+    s.cap_giant = (size_t)s.giant_bytes;
   }
   if (maxP > c->pair_cap) {
     // the pair buffers are the context's, shared by every schedule: the
@@ -323,6 +331,15 @@ LpatFn lane_kernel(int Z) {
 // mixed launch at one workgroup per CU, the small-net blocks using as many
 // group areas as the large tree's LDS already holds.
 int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
+  if (m.cls_n[kClsGiant]) {
+    // nets above kLargeDeg pins: node arrays in the schedule's global scratch
+    TreeArgs G = A;
+    G.list = s.d_list + m.cls_ofs[kClsGiant];
+    G.n_list = m.cls_n[kClsGiant];
+    const int mcap = (int)(((int64_t)c->lds_max - 4096) / 8);
+    hipLaunchKernelGGL(k_tree_giant, dim3(m.cls_n[kClsGiant]), dim3(1024), (size_t)mcap * 8, c->stream, G, mcap,
+                       s.d_giant, m.nslots, m.slot0);
+  }
   SmallClasses C;
   int blocks = 0;
   for (int k = 0; k < 5; ++k) {
@@ -480,6 +497,7 @@ void free_sched(Sched& s) {
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
+  (void)hipFree(s.d_giant);
   s = Sched();
 }
 
@@ -589,6 +607,8 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_tree_mixed<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 2048) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tree_giant, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)c->lds_max - 4096) != hipSuccess ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
@@ -719,6 +739,8 @@ int gr_route_batch_static(gr_ctx* c, const int32_t* net_ids, int32_t n, int64_t*
   int32_t ofs[2] = {0, n};
   int rc = host_sched(c, st, h, 1, ofs, net_ids);
   if (rc) return rc;
+  if (st.giant_bytes)
+    return fail(c, GR_ERANGE, "nets of more than 4096 pins need device memory for their trees (use gr_route_batch)");
   // carve the static array: descriptors, chunk owners, outputs, pair results
   const size_t npos = h.slot.size(), nch = h.chunk.size();
   size_t off = 0;

@@ -151,7 +151,9 @@ int gr_route_batch(gr_ctx* ctx, const int32_t* net_ids, int32_t n,
  * whose buffers do not fit (6 B per pin pair + 4 B per 64 pairs + 12 B per
  * pin + 36 B per net, each part rounded up to 256 B) gets GR_ERANGE before any device work.
  * Exhaustive costing only: GR_ESTATE while a learned proposal or sample
- * capture is set.  Per-kernel timing is not recorded for these calls.
+ * capture is set; nets of more than 4096 pins give GR_ERANGE (their trees
+ * keep node arrays in device memory).  Per-kernel timing is not recorded for
+ * these calls.
  * Every call on one GPU uses that GPU's one array, which keeps the last
  * call's contents (nothing in it needs clearing: a call writes every byte it
  * reads); calls that overlap on one GPU, from any context, race.
@@ -165,9 +167,10 @@ int64_t gr_static_scratch_bytes(void);
 /*
  * Keep a whole batch schedule resident on the device: batch b covers
  * net_ids[batch_ofst[b] .. batch_ofst[b+1]).  Replaces the batcher's per-batch
- * uploads and scans (part_net/net_part.hpp:345-369).  A routed net may have
- * at most 4096 pins (its tree is built in one workgroup's LDS); larger nets
- * give GR_ERANGE here and in gr_route_batch.
+ * uploads and scans (part_net/net_part.hpp:345-369).  Nets of up to 4096 pins
+ * build their trees in one workgroup's LDS; larger ones (up to the 32767 the
+ * 15-bit pin index allows, mst_common.h:25) keep their node arrays in device
+ * memory the schedule owns (44 B per output slot of the batch).
  */
 int gr_set_schedule(gr_ctx* ctx, int32_t n_batches, const int32_t* batch_ofst,
                     const int32_t* net_ids);

@@ -140,3 +140,36 @@ def test_full_workload_sampled_batches(gr, orc, gpu, kind, X, nets):
     assert connected_components(g, directed=False)[0] == len(ids)
     if kind == 5:  # the stress grid really overflows
         assert int(np.maximum(r.demand() - cap, 0).sum()) > 0
+
+
+def test_giant_nets(gr, orc, gpu):
+    """Nets of more than 4096 pins (up to the 32767 of the 15-bit pin index):
+    their trees keep the node arrays in device memory.  Two such nets in
+    disjoint boxes route in one batch, then one alone, each bit-exact with the
+    oracle; a smaller net follows to check the state."""
+    X, Y, Z = 300, 160, 4
+    cap = gr.gen_grid(2, 21, X, Y, Z)
+    rng = np.random.default_rng(21)
+    nets = []
+    for npins, x0 in ((4300, 0), (5200, 150), (4097, 40), (300, 0)):
+        s = set()
+        while len(s) < npins:
+            s.add(_enc(X, Y, int(rng.integers(0, 2)), int(rng.integers(x0, x0 + 140)), int(rng.integers(0, Y))))
+        nets.append(sorted(s))
+    pfx = np.cumsum([0] + [len(n) for n in nets]).astype(np.int64)
+    pins = np.concatenate([np.array(n, np.int32) for n in nets])
+    r = gr.Router(X, Y, Z, cap)
+    r.set_nets(pfx, pins)
+    o = orc.Oracle(X, Y, Z, cap)
+    o.set_nets(pfx, pins)
+    for batch in ([0, 1], [2], [3]):
+        print(f"giant batch {batch} pins {[len(nets[i]) for i in batch]}", flush=True)
+        e1, l1, c1 = r.route_batch(batch)
+        e2, l2, c2 = o.route_batch(batch)
+        pc1, pp1 = r.pair_costs()
+        pc2, pp2 = o.pair_costs()
+        assert _same_f32(pc1, pc2) and np.array_equal(pp1, pp2), batch
+        assert np.array_equal(e1, e2) and np.array_equal(l1, l2) and _same_f32(c1, c2), batch
+        assert np.array_equal(r.demand(), o.demand()), batch
+    with pytest.raises(gr.RouterError, match="4096"):
+        r.route_batch_static([2])
