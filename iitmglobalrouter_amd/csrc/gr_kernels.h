@@ -8,8 +8,7 @@
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
-//   vterm[cell]      fp32   Q(pos) - Q(pos-1): the via cost's single-cell term
-//                           (bit_util.hpp:108-117), one load instead of two
+//                           (one float of padding follows the last cell)
 //   mark[cell]       uint8  1 from the commit that used the cell until the
 //                           same batch's refresh consumes it
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
@@ -71,16 +70,20 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
                                                  float* __restrict__ remf, int model,
-                                                 float* __restrict__ vterm, int16_t* __restrict__ remz) {
+                                                 int16_t* __restrict__ remz) {
   extern __shared__ float sh[];
+  // lowest marked position of the run, in two slots that alternate between
+  // the runs a workgroup processes: a slot is cleared one run before its use
+  __shared__ int first_hit[2];
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* F = sh + kLut;
-  float* R = F + M;  // the run's prefix, for the single-cell terms
   if (!FLT) {
     for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
-    __syncthreads();
   }
+  if (threadIdx.x < 2) first_hit[threadIdx.x] = 0x7fffffff;
+  __syncthreads();
+  int slot = 0;
   const int lane = threadIdx.x & 63;
   const int nr = g.Z * M;
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -90,6 +93,8 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     if (!all && run_stamp[r] != bstamp) continue;
     const int L = (z & 1) ? g.Y : g.X;
     const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
+    if (threadIdx.x == 0) first_hit[slot ^ 1] = 0x7fffffff;  // read last by the previous run
+    int my_first = all ? 0 : 0x7fffffff;
     for (int c0 = 0; c0 < L; c0 += 1024) {
       const int t4 = c0 + 4 * (int)threadIdx.x;
       float v[4];
@@ -117,7 +122,10 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
             c = slut[vv - kRMin];
             if (hit && remz) remz[((z & 1) ? (uint32_t)i * g.X + line : (uint32_t)line * g.X + i) * 16 + z] = (int16_t)vv;
           }
-          if (hit) mark[base + i] = 0;
+          if (hit) {
+            mark[base + i] = 0;
+            my_first = min(my_first, i);
+          }
         }
         v[q] = c;
       }
@@ -139,6 +147,13 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
       if (t4 + 2 < L) F[t4 + 2] = v[2];
       if (t4 + 3 < L) F[t4 + 3] = top;
     }
+    my_first = min(my_first, __shfl_xor(my_first, 1, 64));
+    my_first = min(my_first, __shfl_xor(my_first, 2, 64));
+    my_first = min(my_first, __shfl_xor(my_first, 4, 64));
+    my_first = min(my_first, __shfl_xor(my_first, 8, 64));
+    my_first = min(my_first, __shfl_xor(my_first, 16, 64));
+    my_first = min(my_first, __shfl_xor(my_first, 32, 64));
+    if (lane == 0 && my_first < 0x7fffffff) atomicMin(&first_hit[slot], my_first);
     __syncthreads();
     // nodes of 512 cells and more: F[i-1] = F[i - 2^(k-1) - 1] + F[i-1],
     // in place, one level per barrier
@@ -147,7 +162,11 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
       for (int i = (threadIdx.x + 1) << k; i <= L; i += (int)blockDim.x << k) F[i - 1] = F[i - 1 - h] + F[i - 1];
       __syncthreads();
     }
-    for (int x = threadIdx.x; x < L; x += blockDim.x) {
+    // Q(x) changes only from the lowest marked cell on: positions below it
+    // keep their stored values
+    const int f = first_hit[slot];
+    slot ^= 1;
+    for (int x = f + threadIdx.x; x < L; x += blockDim.x) {
       const uint32_t i1 = (uint32_t)x + 1;
       float res = 0.0f;
 #pragma unroll
@@ -157,13 +176,8 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
         res += nv;
       }
       pfx[base + x] = res;
-      R[x] = res;
     }
-    __syncthreads();
-    // single-cell term of the reference's via cost, Q(pos) - Q(pos - 1) with
-    // Q(-1) = 0 (bit_util.hpp:108-117), the same fp32 subtraction
-    for (int x = threadIdx.x; x < L; x += blockDim.x) vterm[base + x] = R[x] - (x ? R[x - 1] : 0.0f);
-    __syncthreads();
+    __syncthreads();  // F is rewritten by the next run
   }
 }
 
@@ -174,7 +188,6 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
-                                                   const float* __restrict__ vterm,
                                                    const int32_t* __restrict__ pins,
                                                    const int64_t* __restrict__ pinofs,
                                                    const int64_t* __restrict__ bpair,
@@ -192,7 +205,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
-  const uint64_t best = lane_best<ZT>(g, pfx, vterm, z1, x1, y1, z2, x2, y2);
+  const uint64_t best = lane_best<ZT>(g, pfx, z1, x1, y1, z2, x2, y2);
   out_cost[w] = key_cost((uint32_t)(best >> 32));
   out_pat[w] = (int16_t)(uint32_t)best;
 }

@@ -568,7 +568,6 @@ struct PropArgs {
   const int16_t* remz;  // per-GCELL layer copy of rem (k_remz_build; kept by the refresh)
   const float* loglut;
   const float* pfx;
-  const float* vterm;
   const int32_t* pins;
   const int64_t* pinofs;
   const int64_t* bpair;
@@ -824,7 +823,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     const int pc = pr < nrow ? pr : nrow - 1;
     const int z1 = sc[pc][0], x1 = sc[pc][1], y1 = sc[pc][2], z2 = sc[pc][3], x2 = sc[pc][4], y2 = sc[pc][5];
     skey[par * TT + pr] =
-        lane_best<ZT, true>(g, A.pfx, A.vterm, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0);
+        lane_best<ZT, true>(g, A.pfx, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0);
   }
   __syncthreads();
   if (tid < nrow) {
@@ -833,7 +832,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     bool fb = sfb[tid] != 0;
     if (best == ~0ull) {
       fb = true;
-      best = lane_best<ZT>(g, A.pfx, A.vterm, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5]);
+      best = lane_best<ZT>(g, A.pfx, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5]);
     }
     A.out_cost[t0 + tid] = key_cost((uint32_t)(best >> 32));
     A.out_pat[t0 + tid] = (int16_t)(uint32_t)best;
@@ -1031,7 +1030,6 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.remz = c->d_remz;
     A.loglut = m->d_loglut;
     A.pfx = c->d_pfx;
-    A.vterm = c->d_vterm;
     A.pins = c->d_pins;
     A.pinofs = s.d_pinofs + bm.begin;
     A.bpair = s.d_pair + bm.begin;

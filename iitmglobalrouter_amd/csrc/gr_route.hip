@@ -106,7 +106,7 @@ struct KTimer {
   }
 };
 
-size_t refresh_lds(const gr_ctx* c) { return (kLut + 2 * (size_t)c->M) * sizeof(float); }
+size_t refresh_lds(const gr_ctx* c) { return (kLut + (size_t)c->M) * sizeof(float); }
 size_t refresh_prefix_lds(const gr_ctx* c) { return (3 * (size_t)c->M + 128) * sizeof(float); }
 // Refresh grid: the tuned size, or fewer workgroups when the batch can touch
 // fewer runs (runs < 0: every run, as at set-up and reset).  Workgroups scan
@@ -125,11 +125,11 @@ void launch_refresh(gr_ctx* c, int all, int runs = -1) {
   else if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->d_vterm, (int16_t*)nullptr);
+                       (int16_t*)nullptr);
   else
     hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
                        c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->d_vterm, c->remz_live ? c->d_remz : (int16_t*)nullptr);
+                       c->remz_live ? c->d_remz : (int16_t*)nullptr);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -274,8 +274,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   return upload_sched(c, s, h);
 }
 
-typedef void (*LpatFn)(Geo, const float*, const float*, const int32_t*, const int64_t*, const int64_t*,
-                       const int32_t*, int, int64_t, float*, int16_t*, int32_t*);
+typedef void (*LpatFn)(Geo, const float*, const int32_t*, const int64_t*, const int64_t*, const int32_t*, int,
+                       int64_t, float*, int16_t*, int32_t*);
 
 // Workgroup size of the lane costing kernel for a batch of P pairs: the
 // tuned size, or with 0 the largest of 256, 128, 64 that still gives every
@@ -443,7 +443,7 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     if (fn) {
       const int lb = lpat_block_for(c->tune.lpat_block, m.P);
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
-                         c->d_vterm, c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
+                         c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
                          pcost, ppat, c->d_ctr);
     } else {
       hipLaunchKernelGGL(k_lpat_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
@@ -567,10 +567,10 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
     return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
-      (rc = dalloc(c, &c->d_pfx, cs)) || (rc = dalloc(c, &c->d_vterm, cs)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
+      (rc = dalloc(c, &c->d_pfx, cs + 1)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
       ((capf || c->prefix) && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs)))) ||
-      (c->prefix && (rc = dalloc(c, &c->d_vhalf, cs)))) {
+      (c->prefix && ((rc = dalloc(c, &c->d_vhalf, cs)) || (rc = dalloc(c, &c->d_vterm, cs))))) {
     g_err = c->err;
     return bail(rc);
   }
@@ -612,6 +612,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
+      hipMemset(c->d_pfx + cs, 0, sizeof(float)) != hipSuccess ||
       (c->d_vhalf && hipMemset(c->d_vhalf, 0, sizeof(int32_t) * cs) != hipSuccess) ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
     g_err = "device set-up failed";
@@ -906,7 +907,22 @@ int gr_get_prefix(gr_ctx* c, float* out) {
 int gr_get_via_terms(gr_ctx* c, float* out) {
   if (!c || !out) return fail(c, GR_EINVAL, "null argument");
   HIPC(c, hipStreamSynchronize(c->stream));
-  HIPC(c, hipMemcpy(out, c->d_vterm, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
+  if (c->d_vterm) {
+    HIPC(c, hipMemcpy(out, c->d_vterm, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  // BIT modes: Q(pos) - Q(pos - 1) of the stored prefix, the fp32
+  // subtraction the costing kernels make
+  std::vector<float> q((size_t)c->cs);
+  HIPC(c, hipMemcpy(q.data(), c->d_pfx, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
+  for (int z = 0; z < c->g.Z; ++z) {
+    const int L = (z & 1) ? c->g.Y : c->g.X, lines = (z & 1) ? c->g.X : c->g.Y;
+    for (int l = 0; l < lines; ++l) {
+      const size_t b = (size_t)z * c->g.XY + (size_t)l * L;
+      out[b] = q[b];
+      for (int x = 1; x < L; ++x) out[b + x] = q[b + x] - q[b + x - 1];
+    }
+  }
   return 0;
 }
 
