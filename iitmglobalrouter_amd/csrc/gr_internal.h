@@ -41,6 +41,7 @@ struct BatchMeta {
   int64_t max_large_P;    // most pin pairs of a large net in the batch
   int32_t max_giant;      // most pins of a net above kLargeDeg in the batch (0: none)
   int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
+  bool streams;           // the large nets' pair keys do not fit in the trees' LDS
 };
 
 struct Sched {
@@ -57,8 +58,14 @@ struct Sched {
   int32_t* d_lpat = nullptr;
   float* d_cost = nullptr;
   uint8_t* d_giant = nullptr;  // node arrays of the nets above kLargeDeg pins: 44 B per slot of the batch
-  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0;
+  // square key matrices of the large nets of batches whose keys stream: key
+  // of pair (i, k) at d_sq[d_sqofs[pos] + i*n + k] and [.. + k*n + i]
+  // (d_sqofs[pos] = -1: none), written by the costing kernel
+  int64_t* d_sqofs = nullptr;
+  uint32_t* d_sq = nullptr;
+  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0, cap_sq = 0;
   int64_t giant_bytes = 0;     // largest need over the batches with such nets
+  int64_t sq_words = 0;        // largest square-matrix need over the batches
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;
@@ -71,7 +78,7 @@ struct Sched {
 
 // Per-position descriptors of a schedule, computed on the host before upload.
 struct HostSched {
-  std::vector<int64_t> slot, pair, pinofs;
+  std::vector<int64_t> slot, pair, pinofs, sqofs;
   std::vector<int32_t> degv, list, chunk;
   int64_t slots = 0, maxP = 0;
 };

@@ -185,6 +185,24 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // L-pattern costing
 // ---------------------------------------------------------------------------
 
+// Square key matrices of the nets whose trees stream their pair keys from
+// global memory: the pair (j, k) key at key[ofs[pos] + j*n + k] and at
+// [ofs[pos] + k*n + j], so that every pin's keys form one contiguous row
+// (ofs[pos] < 0 or key null: the net has none).
+struct SquareOut {
+  const int64_t* ofs;
+  const int32_t* deg;
+  uint32_t* key;
+};
+__device__ __forceinline__ void square_store(const SquareOut& sq, int pos, int j, int k, uint32_t key) {
+  if (!sq.key) return;
+  const int64_t so = sq.ofs[pos];
+  if (so < 0) return;
+  const int64_t n = sq.deg[pos];
+  sq.key[so + j * n + k] = key;
+  sq.key[so + k * n + j] = key;
+}
+
 // One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
                                                    const int64_t* __restrict__ bpair,
                                                    const int32_t* __restrict__ chunk_net, int n, int64_t P,
                                                    float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
-                                                   int32_t* __restrict__ ctr) {
+                                                   int32_t* __restrict__ ctr, SquareOut sq) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
   const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
@@ -208,6 +226,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   const uint64_t best = lane_best<ZT>(g, pfx, z1, x1, y1, z2, x2, y2);
   out_cost[w] = key_cost((uint32_t)(best >> 32));
   out_pat[w] = (int16_t)(uint32_t)best;
+  square_store(sq, own.pos, j, k, (uint32_t)(best >> 32));
 }
 
 // Any Z (up to 32): one wavefront per pair, one lane per pattern.
@@ -217,7 +236,7 @@ __global__ __launch_bounds__(256) void k_lpat_wave(Geo g, const float* __restric
                                                    const int64_t* __restrict__ bpair,
                                                    const int32_t* __restrict__ chunk_net, int n, int64_t P,
                                                    float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
-                                                   int32_t* __restrict__ ctr) {
+                                                   int32_t* __restrict__ ctr, SquareOut sq) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
   const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -243,6 +262,7 @@ __global__ __launch_bounds__(256) void k_lpat_wave(Geo g, const float* __restric
   if (lane == 0) {
     out_cost[w] = key_cost((uint32_t)(best >> 32));
     out_pat[w] = (int16_t)(uint32_t)best;
+    square_store(sq, pos, j, k, (uint32_t)(best >> 32));
   }
 }
 
@@ -274,6 +294,8 @@ struct TreeArgs {
   int32_t true_mst;  // record each accepted component's minimum pair instead of the root pair
   int32_t prefix;    // non-BIT commit: half-open wire marks and via half charges
   int32_t* vhalf;    // non-BIT commit: via half charges per cell of this batch
+  const int64_t* sqofs;  // batch position -> square key matrix (-1: none; null: no matrices)
+  const uint32_t* sq;
 };
 
 // The edge an accepted candidate of component root r records: the root pair
@@ -606,6 +628,9 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
   // orderable keys the minimum compares (ord_key), converted once here
   const float* cost = gcost;
   const bool staged = P <= (uint32_t)L.scap;
+  // keys that stream from global memory: the net's square matrix when the
+  // costing kernel wrote one (every pin's keys in one contiguous row)
+  const uint32_t* sqk = (!staged && A.sq && A.sqofs[pos] >= 0) ? A.sq + A.sqofs[pos] : nullptr;
   uint32_t* skey = L.skey;
   if (staged) {
     for (uint32_t o = tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
@@ -660,6 +685,29 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
           m = t < m ? t : m;
         }
         if (a < C && part == 0) best[croot_cur[a]] = m;
+      }
+    } else if (sqk) {
+      // node-centric over the square matrix: one wavefront per pin scans the
+      // pin's row (coalesced) into a per-lane minimum, then a wavefront
+      // minimum.  A lane visits increasing k, and the pair index grows with k
+      // along a row (pairs (i, k < i) come before pairs (k > i, i)), so a
+      // strict-less key compare keeps the lowest index among equal keys.
+      for (int i = wv; i < n; i += nwv) {
+        const int ri = root[i];
+        const uint32_t* row = sqk + (size_t)i * (size_t)n;
+        uint32_t mk = ~0u;
+        int mkk = -1;
+#pragma unroll 4
+        for (int k = lane; k < n; k += 64) {
+          const uint32_t key = row[k];
+          const bool take = (root[k] != ri) & (key < mk);
+          mk = take ? key : mk;
+          mkk = take ? k : mkk;
+        }
+        unsigned long long m = ~0ull;
+        if (mkk >= 0) m = ((unsigned long long)mk << 32) | (mkk < i ? tri_index(i, mkk) : tri_index(mkk, i));
+        m = wave_min_u64(m);
+        if (lane == 0 && m != ~0ull) atomicMin(&best[ri], m);
       }
     } else if (!staged) {
       // costs did not fit in LDS: each thread streams a contiguous chunk of
@@ -830,7 +878,20 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       const uint32_t chunk = (P + gsize - 1) / gsize;
       uint32_t o = (uint32_t)tid * chunk;
       const uint32_t oe = o + chunk < P ? o + chunk : P;
-      if (o < oe) {
+      if (sqk) {
+        // the lower half of the square, row by row (coalesced)
+        for (int j = 1 + wv; j < n; j += nwv) {
+          const int cj = L.cid[root[j]];
+          const uint32_t* row = sqk + (size_t)j * (size_t)n;
+          const uint32_t ob = tri_index(j, 0);
+          for (int k = lane; k < j; k += 64) {
+            const int ck = L.cid[root[k]];
+            if (ck != cj)
+              atomicMin(&mcur[cj > ck ? tri_index(cj, ck) : tri_index(ck, cj)],
+                        ((unsigned long long)row[k] << 32) | (ob + (uint32_t)k));
+          }
+        }
+      } else if (o < oe) {
         int j, k;
         pair_decode(o, j, k);
         int cj = L.cid[root[j]];

@@ -141,10 +141,21 @@ int refresh_all(gr_ctx* c) {
   return 0;
 }
 
+// The workgroup trees of a batch keep the pair keys in LDS when the largest
+// large net's fit next to its node arrays; otherwise every large net of the
+// batch streams them from global memory.
+bool tree_streams(const gr_ctx* c, const BatchMeta& m) {
+  if (!m.cls_n[kClsLarge]) return false;
+  const int64_t budget = (int64_t)c->lds_max - 2048;
+  return (int64_t)tree_lds_bytes(m.max_large, 0, (int)std::min<int64_t>(m.max_large_P, 1 << 30)) > budget;
+}
+
 int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32_t* batch_ofst,
                const int32_t* net_ids) {
   const int32_t npos = batch_ofst[n_batches];
   h.slot.assign(npos, 0);
+  h.sqofs.assign(npos, -1);
+  s.sq_words = 0;
   h.pair.assign(npos, 0);
   h.pinofs.assign(npos, 0);
   h.degv.assign(npos, 0);
@@ -206,6 +217,17 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     m.P = P;
     maxP = std::max(maxP, P);
     if (m.max_giant) s.giant_bytes = std::max<int64_t>(s.giant_bytes, m.nslots * 44);
+    m.streams = tree_streams(c, m);
+    if (m.streams) {
+      int64_t words = 0;
+      for (int q = m.cls_ofs[kClsLarge]; q < m.cls_ofs[kClsLarge] + m.cls_n[kClsLarge]; ++q) {
+        const int32_t i = h.list[q];
+        const int64_t d = h.degv[m.begin + i];
+        h.sqofs[m.begin + i] = words;
+        words += d * d;
+      }
+      s.sq_words = std::max(s.sq_words, words);
+    }
   }
   s.n_pos = npos;
   s.slots = slots;
@@ -224,6 +246,8 @@ int upload_sched(gr_ctx* c, Sched& s, const HostSched& h) {
     HIPC(c, hipMemcpyAsync(s.d_slot, h.slot.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(s.d_pair, h.pair.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(s.d_list, h.list.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, c->stream));
+    if (s.d_sqofs)
+      HIPC(c, hipMemcpyAsync(s.d_sqofs, h.sqofs.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
   }
   if (!h.chunk.empty())
     HIPC(c, hipMemcpyAsync(s.d_chunk, h.chunk.data(), sizeof(int32_t) * h.chunk.size(), hipMemcpyHostToDevice,
@@ -246,9 +270,14 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   if (npos > s.cap_pos) {
     if ((rc = dalloc(c, &s.d_pinofs, npos)) || (rc = dalloc(c, &s.d_deg, npos)) ||
         (rc = dalloc(c, &s.d_slot, npos)) || (rc = dalloc(c, &s.d_pair, npos)) ||
-        (rc = dalloc(c, &s.d_list, npos)) || (rc = dalloc(c, &s.d_cost, npos)))
+        (rc = dalloc(c, &s.d_list, npos)) || (rc = dalloc(c, &s.d_cost, npos)) ||
+        (rc = dalloc(c, &s.d_sqofs, npos)))
       return rc;
     s.cap_pos = npos;
+  }
+  if ((size_t)s.sq_words > s.cap_sq) {
+    if ((rc = dalloc(c, &s.d_sq, (size_t)s.sq_words))) return rc;
+    s.cap_sq = (size_t)s.sq_words;
   }
   if ((size_t)slots > s.cap_slots) {
     if ((rc = dalloc(c, &s.d_edges, slots)) || (rc = dalloc(c, &s.d_lpat, slots))) return rc;
@@ -275,7 +304,7 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
 }
 
 typedef void (*LpatFn)(Geo, const float*, const int32_t*, const int64_t*, const int64_t*, const int32_t*, int,
-                       int64_t, float*, int16_t*, int32_t*);
+                       int64_t, float*, int16_t*, int32_t*, SquareOut);
 
 // Workgroup size of the lane costing kernel for a batch of P pairs: the
 // tuned size, or with 0 the largest of 256, 128, 64 that still gives every
@@ -360,7 +389,7 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
   bool mat = false;
   if (n_large) {
     const int64_t budget = (int64_t)c->lds_max - 2048;
-    if ((int64_t)tree_lds_bytes(cap, 0, (int)std::min<int64_t>(m.max_large_P, 1 << 30)) <= budget) {
+    if (!m.streams) {
       scap = (int)m.max_large_P;
     } else {
       mat = true;
@@ -420,6 +449,9 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   if (m.n == 0) return 0;
   float* pcost = s.ext ? s.x_pair_cost : c->d_pair_cost;
   int16_t* ppat = s.ext ? s.x_pair_pat : c->d_pair_pat;
+  // square key matrices: written by the exhaustive BIT costing kernels only
+  SquareOut sq{nullptr, nullptr, nullptr};
+  if (m.streams && s.d_sq && !c->prop && !c->prefix) sq = SquareOut{s.d_sqofs + m.begin, s.d_deg + m.begin, s.d_sq};
   if (m.P > 0 && c->prop) {
     KTimer t(c, GR_K_LPAT);
     const int rc = mlp_enqueue_costing(c, s, m);
@@ -444,11 +476,11 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
       const int lb = lpat_block_for(c->tune.lpat_block, m.P);
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
-                         pcost, ppat, c->d_ctr);
+                         pcost, ppat, c->d_ctr, sq);
     } else {
       hipLaunchKernelGGL(k_lpat_wave, dim3((uint32_t)((m.P + 3) / 4)), dim3(256), 0, c->stream, c->g, c->d_pfx,
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
-                         pcost, ppat, c->d_ctr);
+                         pcost, ppat, c->d_ctr, sq);
     }
   }
   if (m.P > 0 && c->collect) {
@@ -468,6 +500,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.out_edges = s.d_edges;
   A.out_lpat = s.d_lpat;
   A.out_cost = s.d_cost;
+  A.sqofs = sq.ofs;
+  A.sq = sq.key;
   A.mark = c->d_mark;
   A.run_stamp = c->d_run_stamp;
   A.ctr = c->d_ctr;
@@ -497,7 +531,7 @@ void free_sched(Sched& s) {
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
-  (void)hipFree(s.d_giant);
+  (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq);
   s = Sched();
 }
 

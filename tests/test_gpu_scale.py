@@ -173,3 +173,36 @@ def test_giant_nets(gr, orc, gpu):
         assert np.array_equal(r.demand(), o.demand()), batch
     with pytest.raises(gr.RouterError, match="4096"):
         r.route_batch_static([2])
+
+
+def test_streamed_key_layouts_agree(gr, gpu):
+    """A batch whose large nets stream their pair keys: the trees read the
+    square key matrices the lane costing kernel writes; the wavefront costing
+    kernel writes them too; the allocation-free entry point has none and
+    streams the triangular pair costs.  All three route identically."""
+    X, Y, Z = 256, 256, 10
+    cap = gr.gen_grid(2, 13, X, Y, Z)
+    rng = np.random.default_rng(13)
+    nets = []
+    for npins in (1700, 640, 95, 40, 7, 2):
+        s = set()
+        while len(s) < npins:
+            s.add(_enc(X, Y, int(rng.integers(0, 2)), int(rng.integers(0, X)), int(rng.integers(0, Y))))
+        nets.append(sorted(s))
+    pfx = np.cumsum([0] + [len(n) for n in nets]).astype(np.int64)
+    pins = np.concatenate([np.array(n, np.int32) for n in nets])
+    outs = []
+    for form in ("lane", "wave", "static"):
+        print(f"form {form}", flush=True)
+        r = gr.Router(X, Y, Z, cap)
+        if form == "wave":
+            r.set_tuning(lpat_wave=1)
+        r.set_nets(pfx, pins)
+        res = []
+        for batch in ([0, 1, 2, 3, 4, 5], [1, 2], [0]):
+            e, l, c = r.route_batch_static(batch) if form == "static" else r.route_batch(batch)
+            res += [e, l, np.asarray(c, np.float32).view(np.int32), r.demand().copy()]
+        outs.append(res)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
