@@ -186,9 +186,11 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
 // ---------------------------------------------------------------------------
 
 // Square key matrices of the nets whose trees stream their pair keys from
-// global memory: the pair (j, k) key at key[ofs[pos] + j*n + k] and at
-// [ofs[pos] + k*n + j], so that every pin's keys form one contiguous row
-// (ofs[pos] < 0 or key null: the net has none).
+// global memory: the pair (j, k) key at key[ofs[pos] + j*n4 + k] and at
+// [ofs[pos] + k*n4 + j], n4 = n rounded up to a multiple of 4, so that every
+// pin's keys form one contiguous 16-byte aligned row (ofs[pos] < 0 or key
+// null: the net has none).  The diagonal and the padding are never written.
+__host__ __device__ inline int64_t square_stride(int64_t n) { return (n + 3) & ~(int64_t)3; }
 struct SquareOut {
   const int64_t* ofs;
   const int32_t* deg;
@@ -198,9 +200,9 @@ __device__ __forceinline__ void square_store(const SquareOut& sq, int pos, int j
   if (!sq.key) return;
   const int64_t so = sq.ofs[pos];
   if (so < 0) return;
-  const int64_t n = sq.deg[pos];
-  sq.key[so + j * n + k] = key;
-  sq.key[so + k * n + j] = key;
+  const int64_t n4 = square_stride(sq.deg[pos]);
+  sq.key[so + j * n4 + k] = key;
+  sq.key[so + k * n4 + j] = key;
 }
 
 // One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes).
@@ -630,7 +632,8 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
   const bool staged = P <= (uint32_t)L.scap;
   // keys that stream from global memory: the net's square matrix when the
   // costing kernel wrote one (every pin's keys in one contiguous row)
-  const uint32_t* sqk = (!staged && A.sq && A.sqofs[pos] >= 0) ? A.sq + A.sqofs[pos] : nullptr;
+  const uint32_t* sqk = (MAT && !staged && A.sq && A.sqofs[pos] >= 0) ? A.sq + A.sqofs[pos] : nullptr;
+  const int n4 = (int)square_stride(n);
   uint32_t* skey = L.skey;
   if (staged) {
     for (uint32_t o = tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
@@ -688,21 +691,35 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
       }
     } else if (sqk) {
       // node-centric over the square matrix: one wavefront per pin scans the
-      // pin's row (coalesced) into a per-lane minimum, then a wavefront
-      // minimum.  A lane visits increasing k, and the pair index grows with k
-      // along a row (pairs (i, k < i) come before pairs (k > i, i)), so a
-      // strict-less key compare keeps the lowest index among equal keys.
+      // pin's row, 16 bytes per lane and four loads in flight per chunk of
+      // 1024 keys, into a per-lane minimum, then a wavefront minimum.  A lane
+      // visits increasing k, and the pair index grows with k along a row
+      // (pairs (i, k < i) come before pairs (k > i, i)), so a strict-less key
+      // compare keeps the lowest index among equal keys.
       for (int i = wv; i < n; i += nwv) {
         const int ri = root[i];
-        const uint32_t* row = sqk + (size_t)i * (size_t)n;
+        const uint4* row = reinterpret_cast<const uint4*>(sqk + (size_t)i * (size_t)n4);
         uint32_t mk = ~0u;
         int mkk = -1;
-#pragma unroll 4
-        for (int k = lane; k < n; k += 64) {
-          const uint32_t key = row[k];
-          const bool take = (root[k] != ri) & (key < mk);
-          mk = take ? key : mk;
-          mkk = take ? k : mkk;
+        for (int c0 = 0; c0 < n4; c0 += 1024) {
+          uint4 v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k4 = c0 + 4 * (64 * u + lane);
+            v[u] = k4 < n4 ? row[k4 >> 2] : make_uint4(~0u, ~0u, ~0u, ~0u);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k4 = c0 + 4 * (64 * u + lane);
+            const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int k = k4 + q;
+              const bool take = (k < n) && (kv[q] < mk) && (root[k < n ? k : 0] != ri);
+              mk = take ? kv[q] : mk;
+              mkk = take ? k : mkk;
+            }
+          }
         }
         unsigned long long m = ~0ull;
         if (mkk >= 0) m = ((unsigned long long)mk << 32) | (mkk < i ? tri_index(i, mkk) : tri_index(mkk, i));
@@ -882,7 +899,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
         // the lower half of the square, row by row (coalesced)
         for (int j = 1 + wv; j < n; j += nwv) {
           const int cj = L.cid[root[j]];
-          const uint32_t* row = sqk + (size_t)j * (size_t)n;
+          const uint32_t* row = sqk + (size_t)j * (size_t)n4;
           const uint32_t ob = tri_index(j, 0);
           for (int k = lane; k < j; k += 64) {
             const int ck = L.cid[root[k]];

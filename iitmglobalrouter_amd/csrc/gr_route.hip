@@ -224,7 +224,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
         const int32_t i = h.list[q];
         const int64_t d = h.degv[m.begin + i];
         h.sqofs[m.begin + i] = words;
-        words += d * d;
+        words += d * square_stride(d);
       }
       s.sq_words = std::max(s.sq_words, words);
     }
