@@ -78,11 +78,11 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* F = sh + kLut;
-  if (!FLT) {
-    for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
-  }
   if (threadIdx.x < 2) first_hit[threadIdx.x] = 0x7fffffff;
-  __syncthreads();
+  // the cost table goes to LDS at the first run the workgroup processes,
+  // after that run's first cell loads are issued (a workgroup whose runs
+  // are all unchanged loads nothing)
+  bool ready = false;
   int slot = 0;
   const int lane = threadIdx.x & 63;
   const int nr = g.Z * M;
@@ -98,14 +98,30 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     for (int c0 = 0; c0 < L; c0 += 1024) {
       const int t4 = c0 + 4 * (int)threadIdx.x;
       float v[4];
+      bool hits[4];
+      int rv[4];
+      float rfv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = t4 + q;
+        hits[q] = i < L && !all && mark[base + i];
+        if constexpr (FLT) rfv[q] = i < L ? remf[base + i] : 0.0f;
+        else rv[q] = i < L ? rem[base + i] : 0;
+      }
+      if (!ready) {
+        if (!FLT)
+          for (int i = threadIdx.x; i < kLut; i += blockDim.x) slut[i] = lut[i];
+        __syncthreads();
+        ready = true;
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i = t4 + q;
         float c = 0.0f;
         if (i < L) {
-          const bool hit = !all && mark[base + i];
+          const bool hit = hits[q];
           if constexpr (FLT) {
-            float rf = remf[base + i];
+            float rf = rfv[q];
             if (hit) {
               rf -= 1.0f;
               remf[base + i] = rf;
@@ -113,7 +129,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
             }
             c = cost_f32(model, rf);
           } else {
-            int vv = rem[base + i];
+            int vv = rv[q];
             if (hit) {
               vv -= 1;
               rem[base + i] = vv;
