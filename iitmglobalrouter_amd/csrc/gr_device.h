@@ -113,13 +113,14 @@ __device__ __forceinline__ float Q(const float* __restrict__ pfx, const Geo& g, 
 
 // Single-cell range query Q(pos) - Q(pos - 1) of the via cost at cell
 // (z, x, y) (bit_util.hpp:108-117), Q(-1) = 0: both prefix values come from
-// one 8-byte load at the cell before (at the cell itself when pos = 0, whose
-// neighbour is the next cell or the padding float after the last cell).
+// one 8-byte load at the cell before (at run position 0 that value belongs to
+// another run, or to the padding float before the first cell, and is not
+// used; x - 0.0f is x).
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 __device__ __forceinline__ float via_term(const float* __restrict__ pfx, const Geo& g, int z, int x, int y) {
   const int pos = (z & 1) ? y : x;
-  const f32x2_a4 t = *reinterpret_cast<const f32x2_a4*>(pfx + enc(g, z, x, y) - (pos ? 1u : 0u));
-  return pos ? t.y - t.x : t.x;
+  const f32x2_a4 t = *reinterpret_cast<const f32x2_a4*>(pfx + enc(g, z, x, y) - 1);
+  return t.y - (pos ? t.x : 0.0f);
 }
 
 // Via stack cost, bit_util.hpp:108-117: each layer i in [min, max) adds the
@@ -247,10 +248,13 @@ __device__ __forceinline__ constexpr uint64_t parity_patterns(int par) {
   return m;
 }
 
+// vt: the via terms stored per cell (the refresh writes them while a learned
+// proposal is set), or null to form them from the prefix.
 template <int ZT, bool SEL = false>
 __device__ __forceinline__ uint64_t lane_best(const Geo& g, const float* __restrict__ pfx, int z1, int x1, int y1, int z2,
                                               int x2, int y2, uint64_t mask = ~0ull, int par = 0,
-                                              bool full = true) {
+                                              bool full = true, const float* __restrict__ vt = nullptr) {
+  auto via_term_at = [&](int z, int x, int y) { return vt ? vt[enc(g, z, x, y)] : via_term(pfx, g, z, x, y); };
   constexpr int Z2T = (ZT + 1) / 2;
   static_assert(!SEL || (ZT - 1) * Z2T <= 64, "candidate masks hold 64 patterns");
   const bool D = (x1 != x2) && (y1 != y2);
@@ -292,8 +296,8 @@ __device__ __forceinline__ uint64_t lane_best(const Geo& g, const float* __restr
   float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
 #pragma unroll
   for (int i = 0; i < ZT - 1; ++i) {
-    ds[i] = ((nds >> i) & 1) ? via_term(pfx, g, i, x1, y1) : 0.f;
-    dd[i] = ((ndd >> i) & 1) ? via_term(pfx, g, i, x2, y2) : 0.f;
+    ds[i] = ((nds >> i) & 1) ? via_term_at(i, x1, y1) : 0.f;
+    dd[i] = ((ndd >> i) & 1) ? via_term_at(i, x2, y2) : 0.f;
   }
 #pragma unroll
   for (int l = 1; l < ZT; ++l) {
@@ -305,8 +309,8 @@ __device__ __forceinline__ uint64_t lane_best(const Geo& g, const float* __restr
   if (D) {
 #pragma unroll
     for (int i = 0; i < ZT - 1; ++i) {
-      dc1[i] = ((ndc1 >> i) & 1) ? via_term(pfx, g, i, x2, y1) : 0.f;  // corner after a horizontal first wire
-      dc2[i] = ((ndc2 >> i) & 1) ? via_term(pfx, g, i, x1, y2) : 0.f;  // corner after a vertical first wire
+      dc1[i] = ((ndc1 >> i) & 1) ? via_term_at(i, x2, y1) : 0.f;  // corner after a horizontal first wire
+      dc2[i] = ((ndc2 >> i) & 1) ? via_term_at(i, x1, y2) : 0.f;  // corner after a vertical first wire
     }
 #pragma unroll
     for (int l = 1; l < ZT; ++l) {

@@ -102,6 +102,7 @@ struct gr_ctx {
   int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_run_stamp = nullptr;
   uint8_t* d_mark = nullptr;  // per cell: marked by this batch's commit
   float *d_pfx = nullptr, *d_lut = nullptr;
+  float* d_pfx_mem = nullptr;  // the prefix's allocation: one padding float, then d_pfx
   float* d_vterm = nullptr;  // non-BIT mode: per cell via cost of the layer, written by the refresh
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
   float *d_remf = nullptr, *d_capf0 = nullptr;  // fp32 grids: remaining and initial capacity
@@ -137,6 +138,7 @@ struct gr_ctx {
   int16_t* d_remz = nullptr;
   int16_t* d_remz0 = nullptr;
   bool remz_live = false;
+  bool vterm_live = false;  // BIT modes: the refresh also writes d_vterm (a fused proposal reads it)
   // training-data capture after each batch's costing
   gr_mlp* collect = nullptr;
   int collect_per_batch = 0;
@@ -187,5 +189,6 @@ void mlp_detach_all(gr_ctx* c);
 bool mlp_fused(const gr_mlp* m);
 // Drops the captured whole-schedule graph (it holds the old costing set-up).
 void drop_graph(gr_ctx* c);
+int refresh_all_runs(gr_ctx* c);  // rebuild every run's cost state (and the stored via terms, when live)
 
 }  // namespace grh

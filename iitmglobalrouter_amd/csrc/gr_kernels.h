@@ -8,7 +8,7 @@
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
-//                           (one float of padding follows the last cell)
+//                           (one float of padding precedes the first cell)
 //   mark[cell]       uint8  1 from the commit that used the cell until the
 //                           same batch's refresh consumes it
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
@@ -62,7 +62,9 @@ namespace grd {
 //
 // remz (integer grids, or null): the per-GCELL copy of every layer's clamped
 // remaining capacity that the learned proposal's lattice gather reads; a
-// marked cell's entry is rewritten with its new value.
+// marked cell's entry is rewritten with its new value.  vterm (or null): the
+// per-cell via terms the proposal's candidate costing reads, rewritten with
+// the prefix (LDS then holds the run's prefix too).
 template <bool FLT>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  uint8_t* __restrict__ mark, float* __restrict__ pfx,
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
                                                  float* __restrict__ remf, int model,
-                                                 int16_t* __restrict__ remz) {
+                                                 int16_t* __restrict__ remz, float* __restrict__ vterm) {
   extern __shared__ float sh[];
   // lowest marked position of the run, in two slots that alternate between
   // the runs a workgroup processes: a slot is cleared one run before its use
@@ -78,6 +80,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* F = sh + kLut;
+  float* R = F + M;  // the run's prefix, for stored via terms (vterm non-null)
   if (threadIdx.x < 2) first_hit[threadIdx.x] = 0x7fffffff;
   // the cost table goes to LDS at the first run the workgroup processes,
   // after that run's first cell loads are issued (a workgroup whose runs
@@ -182,7 +185,9 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     // keep their stored values
     const int f = first_hit[slot];
     slot ^= 1;
-    for (int x = f + threadIdx.x; x < L; x += blockDim.x) {
+    // with stored via terms, Q(f - 1) is formed again for the first of them
+    const int f0 = (vterm && f > 0) ? f - 1 : f;
+    for (int x = f0 + threadIdx.x; x < L; x += blockDim.x) {
       const uint32_t i1 = (uint32_t)x + 1;
       float res = 0.0f;
 #pragma unroll
@@ -191,9 +196,16 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
         const float nv = ((i1 >> b) & 1u) ? F[ib - 1] : 0.0f;
         res += nv;
       }
-      pfx[base + x] = res;
+      if (x >= f) pfx[base + x] = res;
+      if (vterm) R[x] = res;
     }
-    __syncthreads();  // F is rewritten by the next run
+    if (vterm) {
+      // Q(pos) - Q(pos - 1), Q(-1) = 0 (bit_util.hpp:108-117): the fp32
+      // subtraction the costing kernels make from the prefix
+      __syncthreads();
+      for (int x = f + threadIdx.x; x < L; x += blockDim.x) vterm[base + x] = R[x] - (x ? R[x - 1] : 0.0f);
+    }
+    __syncthreads();  // F (and R) are rewritten by the next run
   }
 }
 

@@ -566,6 +566,7 @@ struct PropArgs {
   Geo g;
   const int32_t* rem;
   const int16_t* remz;  // per-GCELL layer copy of rem (k_remz_build; kept by the refresh)
+  const float* vterm;   // per-cell via terms (kept by the refresh while the proposal is set)
   const float* loglut;
   const float* pfx;
   const int32_t* pins;
@@ -823,7 +824,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     const int pc = pr < nrow ? pr : nrow - 1;
     const int z1 = sc[pc][0], x1 = sc[pc][1], y1 = sc[pc][2], z2 = sc[pc][3], x2 = sc[pc][4], y2 = sc[pc][5];
     skey[par * TT + pr] =
-        lane_best<ZT, true>(g, A.pfx, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0);
+        lane_best<ZT, true>(g, A.pfx, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0, A.vterm);
   }
   __syncthreads();
   if (tid < nrow) {
@@ -832,7 +833,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     bool fb = sfb[tid] != 0;
     if (best == ~0ull) {
       fb = true;
-      best = lane_best<ZT>(g, A.pfx, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5]);
+      best = lane_best<ZT>(g, A.pfx, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5], ~0ull, 0,
+                           true, A.vterm);
     }
     A.out_cost[t0 + tid] = key_cost((uint32_t)(best >> 32));
     A.out_pat[t0 + tid] = (int16_t)(uint32_t)best;
@@ -1001,6 +1003,7 @@ void mlp_detach_all(gr_ctx* c) {
   c->models.clear();
   c->prop = c->collect = nullptr;
   c->remz_live = false;
+  c->vterm_live = false;
 }
 
 // Exhaustive costing already ran; capture a strided sample of the batch's
@@ -1030,6 +1033,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.remz = c->d_remz;
     A.loglut = m->d_loglut;
     A.pfx = c->d_pfx;
+    A.vterm = c->vterm_live ? c->d_vterm : nullptr;
     A.pins = c->d_pins;
     A.pinofs = s.d_pinofs + bm.begin;
     A.bpair = s.d_pair + bm.begin;
@@ -1143,6 +1147,7 @@ void gr_mlp_destroy(gr_mlp* m) {
     if (c->prop == m) {
       c->prop = nullptr;
       c->remz_live = false;
+      c->vterm_live = false;
       drop_graph(c);
     }
     if (c->collect == m) c->collect = nullptr;
@@ -1339,6 +1344,7 @@ int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
   HIPC(c, hipStreamSynchronize(c->stream));
   drop_graph(c);  // the captured pass holds the old costing kernel and its arguments
   c->remz_live = false;
+  c->vterm_live = false;
   if (m && mlp_fused(m)) {  // the lattice gather's per-GCELL layer copy, from here on kept by the refresh
     int rc;
     if (!c->d_remz && ((rc = dalloc(c, &c->d_remz, (size_t)16 * c->g.XY)) ||
@@ -1351,6 +1357,14 @@ int gr_set_proposal(gr_ctx* c, gr_mlp* m, int32_t k, float threshold) {
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
     c->remz_live = true;
+    // the candidate costing reads stored via terms (one 4-byte load each):
+    // from here on the refresh writes them too
+    if (!c->prefix) {
+      if (!c->d_vterm && (rc = dalloc(c, &c->d_vterm, (size_t)c->cs))) return rc;
+      c->vterm_live = true;
+      if ((rc = refresh_all_runs(c))) return rc;
+      HIPC(c, hipStreamSynchronize(c->stream));
+    }
   }
   c->prop = m;
   c->prop_k = k;

@@ -106,7 +106,9 @@ struct KTimer {
   }
 };
 
-size_t refresh_lds(const gr_ctx* c) { return (kLut + (size_t)c->M) * sizeof(float); }
+size_t refresh_lds(const gr_ctx* c, bool with_vterm) {
+  return (kLut + (with_vterm ? 2 : 1) * (size_t)c->M) * sizeof(float);
+}
 size_t refresh_prefix_lds(const gr_ctx* c) { return (3 * (size_t)c->M + 128) * sizeof(float); }
 // Refresh grid: the tuned size, or fewer workgroups when the batch can touch
 // fewer runs (runs < 0: every run, as at set-up and reset).  Workgroups scan
@@ -123,13 +125,14 @@ void launch_refresh(gr_ctx* c, int all, int runs = -1) {
                        c->g, c->M, c->d_remf, c->d_mark, c->d_vhalf, c->d_pfx, c->d_vterm, c->d_run_stamp, c->d_ctr,
                        all);
   else if (c->d_remf)
-    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       (int16_t*)nullptr);
+    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c, false), c->stream,
+                       c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf,
+                       c->model, (int16_t*)nullptr, (float*)nullptr);
   else
-    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c), c->stream, c->g, c->M,
-                       c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->remz_live ? c->d_remz : (int16_t*)nullptr);
+    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c, c->vterm_live),
+                       c->stream, c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all,
+                       c->d_remf, c->model, c->remz_live ? c->d_remz : (int16_t*)nullptr,
+                       c->vterm_live ? c->d_vterm : (float*)nullptr);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -537,6 +540,11 @@ void free_sched(Sched& s) {
 
 }  // namespace
 
+namespace grh {
+int refresh_all_runs(gr_ctx* c) { return refresh_all(c); }
+}  // namespace grh
+
+
 extern "C" {
 
 const char* gr_version(void) { return "iitmglobalrouter_amd 0.2 (gfx950)"; }
@@ -601,18 +609,19 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
     return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
-      (rc = dalloc(c, &c->d_pfx, cs + 1)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
+      (rc = dalloc(c, &c->d_pfx_mem, cs + 1)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
       ((capf || c->prefix) && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs)))) ||
       (c->prefix && ((rc = dalloc(c, &c->d_vhalf, cs)) || (rc = dalloc(c, &c->d_vterm, cs))))) {
     g_err = c->err;
     return bail(rc);
   }
+  c->d_pfx = c->d_pfx_mem + 1;
   c->lds_max = prop.sharedMemPerBlock;
   bool ok = hipFuncSetAttribute((const void*)k_refresh<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)refresh_lds(c)) == hipSuccess &&
+                                (int)refresh_lds(c, true)) == hipSuccess &&
             hipFuncSetAttribute((const void*)k_refresh<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)refresh_lds(c)) == hipSuccess &&
+                                (int)refresh_lds(c, false)) == hipSuccess &&
             hipFuncSetAttribute((const void*)k_refresh_prefix, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)refresh_prefix_lds(c)) == hipSuccess;
   if (c->prefix && refresh_prefix_lds(c) > c->lds_max) ok = false;
@@ -646,7 +655,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
-      hipMemset(c->d_pfx + cs, 0, sizeof(float)) != hipSuccess ||
+      hipMemset(c->d_pfx_mem, 0, sizeof(float)) != hipSuccess ||
       (c->d_vhalf && hipMemset(c->d_vhalf, 0, sizeof(int32_t) * cs) != hipSuccess) ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
     g_err = "device set-up failed";
@@ -678,7 +687,7 @@ void gr_destroy(gr_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   mlp_detach_all(c);
   (void)hipFree(c->d_rem); (void)hipFree(c->d_cap0); (void)hipFree(c->d_mark); (void)hipFree(c->d_run_stamp);
-  (void)hipFree(c->d_pfx); (void)hipFree(c->d_vterm); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
+  (void)hipFree(c->d_pfx_mem); (void)hipFree(c->d_vterm); (void)hipFree(c->d_lut); (void)hipFree(c->d_ctr);
   (void)hipFree(c->d_remf); (void)hipFree(c->d_capf0); (void)hipFree(c->d_vhalf);
   (void)hipFree(c->d_wcnt); (void)hipFree(c->d_vcnt); (void)hipFree(c->d_estamp);
   (void)hipFree(c->d_eval_part); (void)hipFree(c->d_eval_out);
@@ -941,7 +950,7 @@ int gr_get_prefix(gr_ctx* c, float* out) {
 int gr_get_via_terms(gr_ctx* c, float* out) {
   if (!c || !out) return fail(c, GR_EINVAL, "null argument");
   HIPC(c, hipStreamSynchronize(c->stream));
-  if (c->d_vterm) {
+  if (c->prefix) {
     HIPC(c, hipMemcpy(out, c->d_vterm, sizeof(float) * c->cs, hipMemcpyDeviceToHost));
     return 0;
   }
