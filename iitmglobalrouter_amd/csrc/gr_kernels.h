@@ -62,10 +62,10 @@ namespace grd {
 //
 // remz (integer grids, or null): the per-GCELL copy of every layer's clamped
 // remaining capacity that the learned proposal's lattice gather reads; a
-// marked cell's entry is rewritten with its new value.  vterm (or null): the
+// marked cell's entry is rewritten with its new value.  VT: vterm holds the
 // per-cell via terms the proposal's candidate costing reads, rewritten with
 // the prefix (LDS then holds the run's prefix too).
-template <bool FLT>
+template <bool FLT, bool VT = false>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  uint8_t* __restrict__ mark, float* __restrict__ pfx,
                                                  const float* __restrict__ lut,
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     const int f = first_hit[slot];
     slot ^= 1;
     // with stored via terms, Q(f - 1) is formed again for the first of them
-    const int f0 = (vterm && f > 0) ? f - 1 : f;
+    const int f0 = (VT && f > 0) ? f - 1 : f;
     for (int x = f0 + threadIdx.x; x < L; x += blockDim.x) {
       const uint32_t i1 = (uint32_t)x + 1;
       float res = 0.0f;
@@ -196,10 +196,10 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
         const float nv = ((i1 >> b) & 1u) ? F[ib - 1] : 0.0f;
         res += nv;
       }
-      if (x >= f) pfx[base + x] = res;
-      if (vterm) R[x] = res;
+      if (!VT || x >= f) pfx[base + x] = res;
+      if (VT) R[x] = res;
     }
-    if (vterm) {
+    if constexpr (VT) {
       // Q(pos) - Q(pos - 1), Q(-1) = 0 (bit_util.hpp:108-117): the fp32
       // subtraction the costing kernels make from the prefix
       __syncthreads();

@@ -129,10 +129,10 @@ void launch_refresh(gr_ctx* c, int all, int runs = -1) {
                        c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf,
                        c->model, (int16_t*)nullptr, (float*)nullptr);
   else
-    hipLaunchKernelGGL(k_refresh<false>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c, c->vterm_live),
-                       c->stream, c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all,
-                       c->d_remf, c->model, c->remz_live ? c->d_remz : (int16_t*)nullptr,
-                       c->vterm_live ? c->d_vterm : (float*)nullptr);
+    hipLaunchKernelGGL((c->vterm_live ? k_refresh<false, true> : k_refresh<false, false>), dim3(refresh_grid(c, runs)),
+                       dim3(256), refresh_lds(c, c->vterm_live), c->stream, c->g, c->M, c->d_rem, c->d_mark, c->d_pfx,
+                       c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
+                       c->remz_live ? c->d_remz : (int16_t*)nullptr, c->vterm_live ? c->d_vterm : (float*)nullptr);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -618,7 +618,9 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
   }
   c->d_pfx = c->d_pfx_mem + 1;
   c->lds_max = prop.sharedMemPerBlock;
-  bool ok = hipFuncSetAttribute((const void*)k_refresh<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  bool ok = hipFuncSetAttribute((const void*)k_refresh<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)refresh_lds(c, false)) == hipSuccess &&
+            hipFuncSetAttribute((const void*)k_refresh<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)refresh_lds(c, true)) == hipSuccess &&
             hipFuncSetAttribute((const void*)k_refresh<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)refresh_lds(c, false)) == hipSuccess &&
