@@ -9,6 +9,7 @@ constexpr int kRMin = -256;
 constexpr int kLut = 512;  // r in [-256, 255]
 constexpr int kSmallDeg = 64;  // nets up to this many pins: one wavefront per net
 constexpr int kLargeDeg = 4096;  // LDS-resident tree (36 B per pin) for nets up to this many pins
+constexpr int kRowTaskPins = 16;  // pins per workgroup of the first-round row-minimum kernel
 constexpr int kMaxDeg = 32767;   // 15-bit pin index of the edge encoding (mst_common.h:25)
 
 struct Geo {

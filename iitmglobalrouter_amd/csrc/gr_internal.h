@@ -42,6 +42,7 @@ struct BatchMeta {
   int32_t max_giant;      // most pins of a net above kLargeDeg in the batch (0: none)
   int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
   bool streams;           // the large nets' pair keys do not fit in the trees' LDS
+  int32_t rowtask_ofs, n_rowtask;  // first-round row-minimum tasks (16 pins of a square-matrix net each)
 };
 
 struct Sched {
@@ -63,9 +64,12 @@ struct Sched {
   // (d_sqofs[pos] = -1: none), written by the costing kernel
   int64_t* d_sqofs = nullptr;
   uint32_t* d_sq = nullptr;
-  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0, cap_sq = 0;
+  int2* d_rowtask = nullptr;          // (batch position, first pin) of each row-minimum task
+  unsigned long long* d_rowmin = nullptr;  // per output slot: the pin's first-round minimum (key << 32 | pair)
+  size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0, cap_sq = 0, cap_rowtask = 0, cap_rowmin = 0;
   int64_t giant_bytes = 0;     // largest need over the batches with such nets
   int64_t sq_words = 0;        // largest square-matrix need over the batches
+  int64_t rowmin_slots = 0;    // most output slots of a batch with square matrices
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;
@@ -80,6 +84,7 @@ struct Sched {
 struct HostSched {
   std::vector<int64_t> slot, pair, pinofs, sqofs;
   std::vector<int32_t> degv, list, chunk;
+  std::vector<int2> rowtask;
   int64_t slots = 0, maxP = 0;
 };
 

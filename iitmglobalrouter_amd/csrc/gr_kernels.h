@@ -326,6 +326,8 @@ struct TreeArgs {
   int32_t* vhalf;    // non-BIT commit: via half charges per cell of this batch
   const int64_t* sqofs;  // batch position -> square key matrix (-1: none; null: no matrices)
   const uint32_t* sq;
+  const unsigned long long* rowmin;  // per output slot - rowmin_base: first-round pin minima (null: none)
+  int64_t rowmin_base;
 };
 
 // The edge an accepted candidate of component root r records: the root pair
@@ -637,6 +639,64 @@ __device__ __forceinline__ int tree_ids(const TreeLds& L, int n, int tid, int gs
   return total;
 }
 
+// Minimum (key << 32 | pair index) over row i of a square key matrix (n
+// pins, rows of square_stride(n) keys) among the columns k with other(k),
+// for one wavefront: 16 bytes per lane and four loads in flight per chunk of
+// 1024 keys, a per-lane minimum, then a wavefront minimum (~0: none).  A lane
+// visits increasing k, and the pair index grows with k along a row (pairs
+// (i, k < i) come before pairs (k > i, i)), so a strict-less key compare
+// keeps the lowest index among equal keys.
+template <class Other>
+__device__ __forceinline__ unsigned long long square_row_min(const uint32_t* rowp, int n, int i, int lane, Other other) {
+  const int n4 = (int)square_stride(n);
+  const uint4* row = reinterpret_cast<const uint4*>(rowp);
+  uint32_t mk = ~0u;
+  int mkk = -1;
+  for (int c0 = 0; c0 < n4; c0 += 1024) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k4 = c0 + 4 * (64 * u + lane);
+      v[u] = k4 < n4 ? row[k4 >> 2] : make_uint4(~0u, ~0u, ~0u, ~0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k4 = c0 + 4 * (64 * u + lane);
+      const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k4 + q;
+        const bool take = (k < n) && (kv[q] < mk) && other(k < n ? k : 0);
+        mk = take ? kv[q] : mk;
+        mkk = take ? k : mkk;
+      }
+    }
+  }
+  unsigned long long m = ~0ull;
+  if (mkk >= 0) m = ((unsigned long long)mk << 32) | (mkk < i ? tri_index(i, mkk) : tri_index(mkk, i));
+  return wave_min_u64(m);
+}
+
+// First Boruvka round of the nets with square key matrices, over the whole
+// GPU: no component is merged yet, so pin i's candidate is its minimum over
+// the whole row (k != i), exactly as the workgroup tree's round computes it.
+// One wavefront per pin, kRowTaskPins pins of one net per workgroup.
+__global__ __launch_bounds__(64 * kRowTaskPins) void k_row_min(const int2* __restrict__ task,
+                                                               const int32_t* __restrict__ deg,
+                                                               const int64_t* __restrict__ slot,
+                                                               const int64_t* __restrict__ sqofs,
+                                                               const uint32_t* __restrict__ sq,
+                                                               unsigned long long* __restrict__ rowmin,
+                                                               int64_t slot0) {
+  const int2 t = task[blockIdx.x];
+  const int pos = t.x, i = t.y + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int n = deg[pos];
+  if (i >= n) return;  // the whole wavefront
+  const unsigned long long m = square_row_min(sq + sqofs[pos] + (size_t)i * (size_t)square_stride(n), n, i, lane,
+                                              [&](int k) { return k != i; });
+  if (lane == 0) rowmin[slot[pos] - slot0 + i] = m;
+}
+
 template <bool MAT, bool GN = false>
 __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, const TreeLds& L) {
   GR_TREE_MARK_INIT
@@ -717,41 +777,18 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
         }
         if (a < C && part == 0) best[croot_cur[a]] = m;
       }
+    } else if (sqk && round == 0 && A.rowmin) {
+      // the first round's pin minima came from k_row_min (every root is its
+      // own pin, so each minimum is its root's)
+      const unsigned long long* rm = A.rowmin + (A.slot[pos] - A.rowmin_base);
+      for (int i = tid; i < n; i += gsize) best[i] = rm[i];
     } else if (sqk) {
       // node-centric over the square matrix: one wavefront per pin scans the
-      // pin's row, 16 bytes per lane and four loads in flight per chunk of
-      // 1024 keys, into a per-lane minimum, then a wavefront minimum.  A lane
-      // visits increasing k, and the pair index grows with k along a row
-      // (pairs (i, k < i) come before pairs (k > i, i)), so a strict-less key
-      // compare keeps the lowest index among equal keys.
+      // pin's row (square_row_min), then one atomic per pin
       for (int i = wv; i < n; i += nwv) {
         const int ri = root[i];
-        const uint4* row = reinterpret_cast<const uint4*>(sqk + (size_t)i * (size_t)n4);
-        uint32_t mk = ~0u;
-        int mkk = -1;
-        for (int c0 = 0; c0 < n4; c0 += 1024) {
-          uint4 v[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int k4 = c0 + 4 * (64 * u + lane);
-            v[u] = k4 < n4 ? row[k4 >> 2] : make_uint4(~0u, ~0u, ~0u, ~0u);
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int k4 = c0 + 4 * (64 * u + lane);
-            const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int k = k4 + q;
-              const bool take = (k < n) && (kv[q] < mk) && (root[k < n ? k : 0] != ri);
-              mk = take ? kv[q] : mk;
-              mkk = take ? k : mkk;
-            }
-          }
-        }
-        unsigned long long m = ~0ull;
-        if (mkk >= 0) m = ((unsigned long long)mk << 32) | (mkk < i ? tri_index(i, mkk) : tri_index(mkk, i));
-        m = wave_min_u64(m);
+        const unsigned long long m =
+            square_row_min(sqk + (size_t)i * (size_t)n4, n, i, lane, [&](int k) { return root[k] != ri; });
         if (lane == 0 && m != ~0ull) atomicMin(&best[ri], m);
       }
     } else if (!staged) {

@@ -158,7 +158,9 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
   const int32_t npos = batch_ofst[n_batches];
   h.slot.assign(npos, 0);
   h.sqofs.assign(npos, -1);
+  h.rowtask.clear();
   s.sq_words = 0;
+  s.rowmin_slots = 0;
   h.pair.assign(npos, 0);
   h.pinofs.assign(npos, 0);
   h.degv.assign(npos, 0);
@@ -221,6 +223,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     maxP = std::max(maxP, P);
     if (m.max_giant) s.giant_bytes = std::max<int64_t>(s.giant_bytes, m.nslots * 44);
     m.streams = tree_streams(c, m);
+    m.rowtask_ofs = (int32_t)h.rowtask.size();
     if (m.streams) {
       int64_t words = 0;
       for (int q = m.cls_ofs[kClsLarge]; q < m.cls_ofs[kClsLarge] + m.cls_n[kClsLarge]; ++q) {
@@ -228,9 +231,12 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
         const int64_t d = h.degv[m.begin + i];
         h.sqofs[m.begin + i] = words;
         words += d * square_stride(d);
+        for (int r0 = 0; r0 < d; r0 += kRowTaskPins) h.rowtask.push_back(make_int2(i, r0));
       }
       s.sq_words = std::max(s.sq_words, words);
+      s.rowmin_slots = std::max(s.rowmin_slots, m.nslots);
     }
+    m.n_rowtask = (int32_t)h.rowtask.size() - m.rowtask_ofs;
   }
   s.n_pos = npos;
   s.slots = slots;
@@ -252,6 +258,9 @@ int upload_sched(gr_ctx* c, Sched& s, const HostSched& h) {
     if (s.d_sqofs)
       HIPC(c, hipMemcpyAsync(s.d_sqofs, h.sqofs.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, c->stream));
   }
+  if (s.d_rowtask && !h.rowtask.empty())
+    HIPC(c, hipMemcpyAsync(s.d_rowtask, h.rowtask.data(), sizeof(int2) * h.rowtask.size(), hipMemcpyHostToDevice,
+                           c->stream));
   if (!h.chunk.empty())
     HIPC(c, hipMemcpyAsync(s.d_chunk, h.chunk.data(), sizeof(int32_t) * h.chunk.size(), hipMemcpyHostToDevice,
                            c->stream));
@@ -281,6 +290,14 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   if ((size_t)s.sq_words > s.cap_sq) {
     if ((rc = dalloc(c, &s.d_sq, (size_t)s.sq_words))) return rc;
     s.cap_sq = (size_t)s.sq_words;
+  }
+  if (h.rowtask.size() > s.cap_rowtask) {
+    if ((rc = dalloc(c, &s.d_rowtask, h.rowtask.size()))) return rc;
+    s.cap_rowtask = h.rowtask.size();
+  }
+  if ((size_t)s.rowmin_slots > s.cap_rowmin) {
+    if ((rc = dalloc(c, &s.d_rowmin, (size_t)s.rowmin_slots))) return rc;
+    s.cap_rowmin = (size_t)s.rowmin_slots;
   }
   if ((size_t)slots > s.cap_slots) {
     if ((rc = dalloc(c, &s.d_edges, slots)) || (rc = dalloc(c, &s.d_lpat, slots))) return rc;
@@ -505,6 +522,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.out_cost = s.d_cost;
   A.sqofs = sq.ofs;
   A.sq = sq.key;
+  A.rowmin = nullptr;
+  A.rowmin_base = 0;
   A.mark = c->d_mark;
   A.run_stamp = c->d_run_stamp;
   A.ctr = c->d_ctr;
@@ -516,6 +535,15 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.vhalf = c->d_vhalf;
   {
     KTimer t(c, GR_K_TREE);
+    if (sq.key && m.n_rowtask && s.d_rowmin) {
+      // first Boruvka round of the square-matrix nets over the whole GPU:
+      // every pin's minimum over its row (no component is merged yet)
+      hipLaunchKernelGGL(k_row_min, dim3(m.n_rowtask), dim3(64 * kRowTaskPins), 0, c->stream,
+                         s.d_rowtask + m.rowtask_ofs, s.d_deg + m.begin, s.d_slot + m.begin, s.d_sqofs + m.begin,
+                         (const uint32_t*)s.d_sq, s.d_rowmin, m.slot0);
+      A.rowmin = s.d_rowmin;
+      A.rowmin_base = m.slot0;
+    }
     const int rc = launch_trees(c, s, m, A);
     if (rc) return rc;
   }
@@ -534,7 +562,8 @@ void free_sched(Sched& s) {
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
-  (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq);
+  (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq); (void)hipFree(s.d_rowtask);
+  (void)hipFree(s.d_rowmin);
   s = Sched();
 }
 
