@@ -23,9 +23,14 @@ __device__ __forceinline__ uint32_t enc(const Geo& g, int z, int x, int y) {
 }
 __device__ __forceinline__ void dec(const Geo& g, uint32_t v, int& z, int& x, int& y) {
   z = (int)(v / g.XY);
-  uint32_t r = v - (uint32_t)z * g.XY;
-  if (z & 1) { x = (int)(r / g.Y); y = (int)(r - (uint32_t)x * g.Y); }
-  else       { y = (int)(r / g.X); x = (int)(r - (uint32_t)y * g.X); }
+  const uint32_t r = v - (uint32_t)z * g.XY;
+  // one division by the run length; the results are selected by value (a
+  // branch assigning through the references makes the compiler keep them in
+  // scratch memory)
+  const uint32_t L = (z & 1) ? g.Y : g.X;
+  const uint32_t q = r / L, m = r - q * L;
+  x = (int)((z & 1) ? q : m);
+  y = (int)((z & 1) ? m : q);
 }
 
 // e^x in double for cell costs of grids with fp32 capacities.  Cody-Waite
@@ -249,6 +254,92 @@ __device__ __forceinline__ constexpr uint64_t parity_patterns(int par) {
   return m;
 }
 
+// The terms of one pin pair that every pattern is built from: single-cell
+// via terms at the source, destination and the two corners (layers 0..Z-2),
+// and the first / second wire term on each layer 1..Z-1.
+template <int ZT>
+struct PairTerms {
+  float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
+};
+
+// The pattern loop over loaded terms (dc1, dc2 and w2 are read only for bent
+// pairs, w1 only on layers the first wire moves on); mask as in lane_best.
+template <int ZT, bool SEL>
+__device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, int x1, int y1, int z2, int x2,
+                                                int y2, uint64_t mask) {
+  constexpr int Z2T = (ZT + 1) / 2;
+  const bool D = (x1 != x2) && (y1 != y2);
+  // Destination via stack from the last wire layer zs (a compile-time value
+  // in the unrolled loops below) to z2: cells z2..zs-1 going down, zs..z2-1
+  // going up, added in ascending layer order.  ddn[i] = t.dd[i] for i >= z2 and
+  // dup[i] = t.dd[i] for i < z2, +0.0f elsewhere, so each pattern adds ddn[0..zs)
+  // and, only when z2 > zs, dup[zs..).  The running cost is never -0.0, so
+  // adding +0.0f leaves it unchanged and the sums equal the cell-by-cell ones.
+  float ddn[ZT - 1], dup[ZT - 1];
+#pragma unroll
+  for (int i = 0; i < ZT - 1; ++i) {
+    ddn[i] = i >= z2 ? t.dd[i] : 0.f;
+    dup[i] = i < z2 ? t.dd[i] : 0.f;
+  }
+  auto to_dst = [&](float& c, const int zs) {
+#pragma unroll
+    for (int i = 0; i < zs; ++i) c += ddn[i];
+    if (z2 > zs) {
+#pragma unroll
+      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
+    }
+  };
+
+  uint64_t best = ~0ull;
+  float vs = 0.f;  // running source via sum for a > z1
+#pragma unroll
+  for (int a = 1; a < ZT; ++a) {
+    float ca = 0.f;
+    if (a > z1) {
+      vs = (a == z1 + 1) ? t.ds[a - 1] : vs + t.ds[a - 1];
+      ca = vs;
+    } else if (a < z1) {
+#pragma unroll
+      for (int i = 0; i < ZT - 1; ++i)
+        if (i >= a && i < z1) ca += t.ds[i];
+    }
+    const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
+    if (moved1) ca += t.w1[a];
+    if (D) {
+      // second wire on b' (b' = 0 leaves both coordinates to cover: invalid)
+#pragma unroll
+      for (int b = 0; b < Z2T; ++b) {
+        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+        if (bp == 0 || bp >= ZT) continue;
+        if (SEL && !((mask >> ((a - 1) * Z2T + b)) & 1)) continue;  // also skips the other parity
+        float c = ca;
+        const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
+#pragma unroll
+        for (int i = lo; i < hi; ++i) c += (a & 1) ? t.dc2[i] : t.dc1[i];
+        c += t.w2[bp];
+        to_dst(c, bp);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
+        best = key < best ? key : best;
+      }
+    } else {
+      // straight pair: every b gives the same route, b = 0 is the lowest p
+      const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
+      int b0 = 0;
+      if (SEL) {
+        const uint64_t sub = (mask >> ((a - 1) * Z2T)) & ((1ull << Z2T) - 1);
+        b0 = sub ? __builtin_ctzll(sub) : -1;
+      }
+      if (at_dst && b0 >= 0) {
+        float c = ca;
+        to_dst(c, a);
+        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b0);
+        best = key < best ? key : best;
+      }
+    }
+  }
+  return best;
+}
+
 // vt: the via terms stored per cell (the refresh writes them while a learned
 // proposal is set), or null to form them from the prefix.
 template <int ZT, bool SEL = false>
@@ -294,101 +385,106 @@ __device__ __forceinline__ uint64_t lane_best(const Geo& g, const float* __restr
     }
   }
 
-  float ds[ZT - 1], dd[ZT - 1], dc1[ZT - 1], dc2[ZT - 1], w1[ZT], w2[ZT];
+  PairTerms<ZT> t;
 #pragma unroll
   for (int i = 0; i < ZT - 1; ++i) {
-    ds[i] = ((nds >> i) & 1) ? via_term_at(i, x1, y1) : 0.f;
-    dd[i] = ((ndd >> i) & 1) ? via_term_at(i, x2, y2) : 0.f;
+    t.ds[i] = ((nds >> i) & 1) ? via_term_at(i, x1, y1) : 0.f;
+    t.dd[i] = ((ndd >> i) & 1) ? via_term_at(i, x2, y2) : 0.f;
   }
 #pragma unroll
   for (int l = 1; l < ZT; ++l) {
     // first wire on layer l from the source: rows y1 on even layers, columns x1 on odd
     const bool moved = (l & 1) ? (y1 != y2) : (x1 != x2);
-    w1[l] = (moved && ((nw1 >> l) & 1)) ? ((l & 1) ? wterm(pfx, g, l, x1, y1, y2) : wterm(pfx, g, l, y1, x1, x2))
+    t.w1[l] = (moved && ((nw1 >> l) & 1)) ? ((l & 1) ? wterm(pfx, g, l, x1, y1, y2) : wterm(pfx, g, l, y1, x1, x2))
                                         : 0.f;
   }
   if (D) {
 #pragma unroll
     for (int i = 0; i < ZT - 1; ++i) {
-      dc1[i] = ((ndc1 >> i) & 1) ? via_term_at(i, x2, y1) : 0.f;  // corner after a horizontal first wire
-      dc2[i] = ((ndc2 >> i) & 1) ? via_term_at(i, x1, y2) : 0.f;  // corner after a vertical first wire
+      t.dc1[i] = ((ndc1 >> i) & 1) ? via_term_at(i, x2, y1) : 0.f;  // corner after a horizontal first wire
+      t.dc2[i] = ((ndc2 >> i) & 1) ? via_term_at(i, x1, y2) : 0.f;  // corner after a vertical first wire
     }
 #pragma unroll
     for (int l = 1; l < ZT; ++l) {
       // second wire into the destination: columns x2 on odd layers, rows y2 on even
-      w2[l] = ((nw2 >> l) & 1) ? ((l & 1) ? wterm(pfx, g, l, x2, y1, y2) : wterm(pfx, g, l, y2, x1, x2)) : 0.f;
+      t.w2[l] = ((nw2 >> l) & 1) ? ((l & 1) ? wterm(pfx, g, l, x2, y1, y2) : wterm(pfx, g, l, y2, x1, x2)) : 0.f;
     }
   }
+  return lane_select<ZT, SEL>(t, z1, x1, y1, z2, x2, y2, mask);
+}
 
-  // Destination via stack from the last wire layer zs (a compile-time value
-  // in the unrolled loops below) to z2: cells z2..zs-1 going down, zs..z2-1
-  // going up, added in ascending layer order.  ddn[i] = dd[i] for i >= z2 and
-  // dup[i] = dd[i] for i < z2, +0.0f elsewhere, so each pattern adds ddn[0..zs)
-  // and, only when z2 > zs, dup[zs..).  The running cost is never -0.0, so
-  // adding +0.0f leaves it unchanged and the sums equal the cell-by-cell ones.
-  float ddn[ZT - 1], dup[ZT - 1];
+// The prefix pair (Q(pos - 1), Q(pos)) at cell (z, x, y), one 8-byte load;
+// Q(pos - 1) is never used at pos = 0 (it belongs to another run, or to the
+// padding float before the first cell).
+__device__ __forceinline__ f32x2_a4 qpair(const float* __restrict__ pfx, const Geo& g, int z, int x, int y) {
+  return *reinterpret_cast<const f32x2_a4*>(pfx + enc(g, z, x, y) - 1);
+}
+// Single-cell via term Q(pos) - Q(pos - 1), Q(-1) = 0, from its prefix pair.
+__device__ __forceinline__ float via_of(f32x2_a4 t, int pos) { return t.y - (pos ? t.x : 0.0f); }
+// Inclusive wire Q(hi) - Q(lo - 1), Q(-1) = 0, between run positions pu != pv
+// from the prefix pairs at its two ends.
+__device__ __forceinline__ float wire_of(f32x2_a4 u, int pu, f32x2_a4 v, int pv) {
+  return pu < pv ? v.y - (pu ? u.x : 0.0f) : u.y - (pv ? v.x : 0.0f);
+}
+
+// lane_best<ZT> (every pattern, prefix in pfx) with the terms loaded per
+// point instead of per term.  Every term of the pair lies at one of four
+// GCELLs, source s = (x1, y1), destination d = (x2, y2) and the corners
+// c1 = (x2, y1), c2 = (x1, y2), and each needs Q(pos) and Q(pos - 1) at one
+// cell of it on some layer:
+//   via terms at s, d, c1, c2 (layers 0..Z-2);
+//   the first wire on layer l runs from s to c1 (even l, row y1) or to c2
+//     (odd l, column x1): its ends are s's and that corner's cells on l;
+//   the second wire on layer l runs from c2 (even l, row y2) or c1 (odd l,
+//     column x2) to d.
+// So one 8-byte load per point and layer, 4Z for a bent pair and 2Z for a
+// straight one (whose moving wires end at d), replaces 8(Z-1) loads of single
+// terms; every term is formed from the same two prefix values by the same
+// fp32 subtraction, so the results are those of lane_best.
+template <int ZT>
+__device__ __forceinline__ uint64_t lane_best_pts(const Geo& g, const float* __restrict__ pfx, int z1, int x1, int y1,
+                                                  int z2, int x2, int y2) {
+  const bool mx = x1 != x2, my = y1 != y2, D = mx && my;
+  constexpr bool top_odd = ((ZT - 1) & 1) != 0;
+  // the top layer carries no via term: its cells are needed for wires only
+  const bool top_moves = top_odd ? my : mx;
+  f32x2_a4 s[ZT], d[ZT], c1[ZT], c2[ZT];
+#pragma unroll
+  for (int l = 0; l < ZT; ++l) {
+    if (l < ZT - 1 || top_moves) s[l] = qpair(pfx, g, l, x1, y1);
+    if (l < ZT - 1 || top_moves || D) d[l] = qpair(pfx, g, l, x2, y2);
+  }
+  if (D) {
+#pragma unroll
+    for (int l = 0; l < ZT; ++l) {
+      c1[l] = qpair(pfx, g, l, x2, y1);
+      c2[l] = qpair(pfx, g, l, x1, y2);
+    }
+  }
+  PairTerms<ZT> t;
 #pragma unroll
   for (int i = 0; i < ZT - 1; ++i) {
-    ddn[i] = i >= z2 ? dd[i] : 0.f;
-    dup[i] = i < z2 ? dd[i] : 0.f;
+    t.ds[i] = via_of(s[i], (i & 1) ? y1 : x1);
+    t.dd[i] = via_of(d[i], (i & 1) ? y2 : x2);
   }
-  auto to_dst = [&](float& c, const int zs) {
 #pragma unroll
-    for (int i = 0; i < zs; ++i) c += ddn[i];
-    if (z2 > zs) {
+  for (int l = 1; l < ZT; ++l) {
+    if (l & 1) t.w1[l] = my ? wire_of(s[l], y1, D ? c2[l] : d[l], y2) : 0.f;
+    else       t.w1[l] = mx ? wire_of(s[l], x1, D ? c1[l] : d[l], x2) : 0.f;
+  }
+  if (D) {
 #pragma unroll
-      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
+    for (int i = 0; i < ZT - 1; ++i) {
+      t.dc1[i] = via_of(c1[i], (i & 1) ? y1 : x2);
+      t.dc2[i] = via_of(c2[i], (i & 1) ? y2 : x1);
     }
-  };
-
-  uint64_t best = ~0ull;
-  float vs = 0.f;  // running source via sum for a > z1
 #pragma unroll
-  for (int a = 1; a < ZT; ++a) {
-    float ca = 0.f;
-    if (a > z1) {
-      vs = (a == z1 + 1) ? ds[a - 1] : vs + ds[a - 1];
-      ca = vs;
-    } else if (a < z1) {
-#pragma unroll
-      for (int i = 0; i < ZT - 1; ++i)
-        if (i >= a && i < z1) ca += ds[i];
-    }
-    const bool moved1 = (a & 1) ? (y1 != y2) : (x1 != x2);
-    if (moved1) ca += w1[a];
-    if (D) {
-      // second wire on b' (b' = 0 leaves both coordinates to cover: invalid)
-#pragma unroll
-      for (int b = 0; b < Z2T; ++b) {
-        const int bp = (a & 1) ? 2 * b : 2 * b + 1;
-        if (bp == 0 || bp >= ZT) continue;
-        if (SEL && !((mask >> ((a - 1) * Z2T + b)) & 1)) continue;  // also skips the other parity
-        float c = ca;
-        const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
-#pragma unroll
-        for (int i = lo; i < hi; ++i) c += (a & 1) ? dc2[i] : dc1[i];
-        c += w2[bp];
-        to_dst(c, bp);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
-        best = key < best ? key : best;
-      }
-    } else {
-      // straight pair: every b gives the same route, b = 0 is the lowest p
-      const bool at_dst = (a & 1) ? (x1 == x2) : (y1 == y2);
-      int b0 = 0;
-      if (SEL) {
-        const uint64_t sub = (mask >> ((a - 1) * Z2T)) & ((1ull << Z2T) - 1);
-        b0 = sub ? __builtin_ctzll(sub) : -1;
-      }
-      if (at_dst && b0 >= 0) {
-        float c = ca;
-        to_dst(c, a);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b0);
-        best = key < best ? key : best;
-      }
+    for (int l = 1; l < ZT; ++l) {
+      if (l & 1) t.w2[l] = wire_of(c1[l], y1, d[l], y2);  // column x2
+      else       t.w2[l] = wire_of(c2[l], x1, d[l], x2);  // row y2
     }
   }
-  return best;
+  return lane_select<ZT, false>(t, z1, x1, y1, z2, x2, y2, ~0ull);
 }
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {

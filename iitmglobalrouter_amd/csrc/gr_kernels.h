@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
-  const uint64_t best = lane_best<ZT>(g, pfx, z1, x1, y1, z2, x2, y2);
+  const uint64_t best = lane_best_pts<ZT>(g, pfx, z1, x1, y1, z2, x2, y2);
   out_cost[w] = key_cost((uint32_t)(best >> 32));
   out_pat[w] = (int16_t)(uint32_t)best;
   square_store(sq, own.pos, j, k, (uint32_t)(best >> 32));
@@ -318,7 +318,6 @@ struct TreeArgs {
   uint8_t* mark;
   int32_t* run_stamp;
   const int32_t* ctr;  // device batch counter; its value is the batch stamp
-  int32_t bstamp;      // filled from *ctr at kernel entry
   int32_t quirk;
   int32_t M;  // run_stamp row stride: max(X, Y)
   int32_t true_mst;  // record each accepted component's minimum pair instead of the root pair
@@ -344,27 +343,27 @@ __device__ __forceinline__ int2 recorded_edge(const TreeArgs& A, const int* root
 // Commit marks are idempotent plain stores: the cell's mark byte and its
 // run's stamp.  The demand update itself happens once per marked cell in
 // k_refresh, which also clears the mark.
-__device__ __forceinline__ void mark_run(const TreeArgs& A, int z, int line, int pos) {
+__device__ __forceinline__ void mark_run(const TreeArgs& A, int32_t bs, int z, int line, int pos) {
   (void)pos;
-  A.run_stamp[z * A.M + line] = A.bstamp;
+  A.run_stamp[z * A.M + line] = bs;
 }
-__device__ __forceinline__ void mark_any(const TreeArgs& A, uint32_t cell) {
+__device__ __forceinline__ void mark_any(const TreeArgs& A, int32_t bs, uint32_t cell) {
   A.mark[cell] = 1;
   int z, x, y;
   dec(A.g, cell, z, x, y);
-  mark_run(A, z, (z & 1) ? x : y, (z & 1) ? y : x);
+  mark_run(A, bs, z, (z & 1) ? x : y, (z & 1) ? y : x);
 }
-__device__ __forceinline__ void mark_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
+__device__ __forceinline__ void mark_wire(const TreeArgs& A, int32_t bs, int z, int line, int lo, int hi) {
   const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
   const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
   for (int t = lo; t <= hi; ++t) A.mark[base + t] = 1;
-  mark_run(A, z, line, lo);
+  mark_run(A, bs, z, line, lo);
 }
-__device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int x, int y) {
+__device__ __forceinline__ void mark_via(const TreeArgs& A, int32_t bs, int za, int zb, int x, int y) {
   const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
   for (int l = lo + 1; l < hi; ++l) {
     A.mark[enc(A.g, l, x, y)] = 1;
-    mark_run(A, l, (l & 1) ? x : y, (l & 1) ? y : x);
+    mark_run(A, bs, l, (l & 1) ? x : y, (l & 1) ? y : x);
   }
 }
 
@@ -384,14 +383,14 @@ __device__ __forceinline__ void mark_via(const TreeArgs& A, int za, int zb, int 
 // (gpu_commit_via :17-61): 2 to the cell at run position 0, 2 to the
 // previous cell at the last position, else 1 to each.  The refresh applies
 // them: -1.0 once per wired cell, then -0.5 per half.
-__device__ __forceinline__ void prefix_wire(const TreeArgs& A, int z, int line, int lo, int hi) {
+__device__ __forceinline__ void prefix_wire(const TreeArgs& A, int32_t bs, int z, int line, int lo, int hi) {
   if (lo >= hi) return;
   const uint32_t L = (z & 1) ? A.g.Y : A.g.X;
   const uint32_t base = (uint32_t)z * A.g.XY + (uint32_t)line * L;
   for (int t = lo; t < hi; ++t) A.mark[base + t] = 1;
-  A.run_stamp[z * A.M + line] = A.bstamp;
+  A.run_stamp[z * A.M + line] = bs;
 }
-__device__ __forceinline__ void prefix_via(const TreeArgs& A, int za, int zb, int x, int y) {
+__device__ __forceinline__ void prefix_via(const TreeArgs& A, int32_t bs, int za, int zb, int x, int y) {
   const int lo = za < zb ? za : zb, hi = za < zb ? zb : za;
   for (int i = lo; i < hi; ++i) {
     const int pos = (i & 1) ? y : x, L = (i & 1) ? A.g.Y : A.g.X;
@@ -399,10 +398,10 @@ __device__ __forceinline__ void prefix_via(const TreeArgs& A, int za, int zb, in
     if (pos == 0) atomicAdd(&A.vhalf[c], 2);
     else if (pos == L - 1) atomicAdd(&A.vhalf[c - 1], 2);
     else { atomicAdd(&A.vhalf[c - 1], 1); atomicAdd(&A.vhalf[c], 1); }
-    A.run_stamp[i * A.M + ((i & 1) ? x : y)] = A.bstamp;
+    A.run_stamp[i * A.M + ((i & 1) ? x : y)] = bs;
   }
 }
-__device__ void commit_edge_prefix(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
+__device__ __forceinline__ void commit_edge_prefix(const TreeArgs& A, int32_t bs, uint32_t src, uint32_t dst, int p) {
   const Geo& g = A.g;
   int z1, x1, y1, z2, x2, y2;
   dec(g, src, z1, x1, y1);
@@ -411,30 +410,30 @@ __device__ void commit_edge_prefix(const TreeArgs& A, uint32_t src, uint32_t dst
   const int bp = (a & 1) ? 2 * b : 2 * b + 1;
   // wires
   int x = x1, y = y1;
-  if (a & 1) { prefix_wire(A, a, x, min(y, y2), max(y, y2)); y = y2; }
-  else       { prefix_wire(A, a, y, min(x, x2), max(x, x2)); x = x2; }
+  if (a & 1) { prefix_wire(A, bs, a, x, min(y, y2), max(y, y2)); y = y2; }
+  else       { prefix_wire(A, bs, a, y, min(x, x2), max(x, x2)); x = x2; }
   if (bp > 0) {
-    if (bp & 1) prefix_wire(A, bp, x, min(y, y2), max(y, y2));
-    else        prefix_wire(A, bp, y, min(x, x2), max(x, x2));
+    if (bp & 1) prefix_wire(A, bs, bp, x, min(y, y2), max(y, y2));
+    else        prefix_wire(A, bs, bp, y, min(x, x2), max(x, x2));
   }
   // vias
   x = x1; y = y1;
-  prefix_via(A, z1, a, x, y);
+  prefix_via(A, bs, z1, a, x, y);
   int z = a;
   if (z & 1) y = y2; else x = x2;
   if (bp > 0) {
-    prefix_via(A, z, bp, x, y);
+    prefix_via(A, bs, z, bp, x, y);
     z = bp;
     if (z & 1) y = y2; else x = x2;
   }
-  if (z != z2) prefix_via(A, z, z2, x, y);
+  if (z != z2) prefix_via(A, bs, z, z2, x, y);
 }
 
-__device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p) {
+__device__ __forceinline__ void commit_edge(const TreeArgs& A, int32_t bs, uint32_t src, uint32_t dst, int p) {
   const Geo& g = A.g;
   if (p < g.Z2 || p >= g.Z * g.Z2) return;  // no valid pattern (cannot happen for Z >= 3)
   if (A.prefix) {
-    commit_edge_prefix(A, src, dst, p);
+    commit_edge_prefix(A, bs, src, dst, p);
     return;
   }
   int z1, x1, y1, z2, x2, y2;
@@ -444,35 +443,35 @@ __device__ void commit_edge(const TreeArgs& A, uint32_t src, uint32_t dst, int p
   const bool D = (x1 != x2) && (y1 != y2);
   int z = z1, x = x1, y = y1, k = 0;
   if (z != a) {
-    mark_via(A, z, a, x, y);
+    mark_via(A, bs, z, a, x, y);
     k = (z < a ? a - z : z - a) - 1;
     z = a;
   }
   if (z & 1) {
     const int lo = min(y, y2), hi = max(y, y2);
-    mark_wire(A, z, x, lo, hi);
+    mark_wire(A, bs, z, x, lo, hi);
     if (A.quirk) {
       const uint32_t e0 = enc(g, z, x, lo);
-      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, e0 - t);
+      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, bs, e0 - t);
     }
     y = y2;
   } else {
     const int lo = min(x, x2), hi = max(x, x2);
-    mark_wire(A, z, y, lo, hi);
+    mark_wire(A, bs, z, y, lo, hi);
     if (A.quirk) {
       const uint32_t e0 = enc(g, z, lo, y);
-      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, e0 - t);
+      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, bs, e0 - t);
     }
     x = x2;
   }
   const int bp = (z & 1) ? 2 * b : 2 * b + 1;
   if (D && bp != 0) {
-    mark_via(A, z, bp, x, y);
+    mark_via(A, bs, z, bp, x, y);
     z = bp;
-    if (z & 1) { mark_wire(A, z, x, min(y, y2), max(y, y2)); y = y2; }
-    else       { mark_wire(A, z, y, min(x, x2), max(x, x2)); x = x2; }
+    if (z & 1) { mark_wire(A, bs, z, x, min(y, y2), max(y, y2)); y = y2; }
+    else       { mark_wire(A, bs, z, y, min(x, x2), max(x, x2)); x = x2; }
   }
-  if (z != z2) mark_via(A, z, z2, x, y);
+  if (z != z2) mark_via(A, bs, z, z2, x, y);
 }
 
 // Net cost: the n-1 accepted minimum costs added in slot order (the
@@ -492,24 +491,24 @@ __device__ __forceinline__ float sum_edge_costs(const float* ecost, int cnt) {
   return nc;
 }
 
-__device__ __forceinline__ void emit_edge(const TreeArgs& A, int pos, int64_t sb, int e, int on1, int on2,
+__device__ __forceinline__ void emit_edge(const TreeArgs& A, int32_t bs, int pos, int64_t sb, int e, int on1, int on2,
                                           const int16_t* pat, const int32_t* npins) {
   A.out_edges[sb + e] = ((int64_t)pos << 30) | ((int64_t)on1 << 15) | (int64_t)on2;
   const int hi = on1 > on2 ? on1 : on2, lo = on1 > on2 ? on2 : on1;
   const int p = pat[(uint32_t)((int64_t)hi * (hi - 1) / 2 + lo)];
   A.out_lpat[sb + e] = p;
-  commit_edge(A, (uint32_t)npins[hi], (uint32_t)npins[lo], p);
+  commit_edge(A, bs, (uint32_t)npins[hi], (uint32_t)npins[lo], p);
 }
 
 // 2-pin nets: the only pair is (1, 0); Boruvka's single round
 // has the mutual candidates 0 -> 1 and 1 -> 0, and root 0 is processed first,
 // so the edge is (0, 1) and the tree cost is 0 + cost (in true-MST mode the
 // minimum pair's endpoints are the roots themselves: the same edge).
-__device__ void tree2_commit(const TreeArgs& A, int pos) {
+__device__ __forceinline__ void tree2_commit(const TreeArgs& A, int32_t bs, int pos) {
   const int64_t pb = A.pinofs[pos];
   const int64_t sb = A.slot[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
-  emit_edge(A, pos, sb, 0, 0, 1, pat, A.pins + pb);
+  emit_edge(A, bs, pos, sb, 0, 0, 1, pat, A.pins + pb);
   A.out_edges[sb + 1] = -1;
   A.out_lpat[sb + 1] = -1;
   A.out_cost[A.cost_base + pos] = 0.0f + A.pair_cost[A.pair[pos]];
@@ -698,7 +697,7 @@ __global__ __launch_bounds__(64 * kRowTaskPins) void k_row_min(const int2* __res
 }
 
 template <bool MAT, bool GN = false>
-__device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, const TreeLds& L) {
+__device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int pos, int tid, int gsize, const TreeLds& L) {
   GR_TREE_MARK_INIT
   GR_TREE_MARK(0);
   int* root = L.root;
@@ -997,7 +996,7 @@ __device__ void tree_commit(const TreeArgs& A, int pos, int tid, int gsize, cons
   }
   const int64_t sb = A.slot[pos];
   const int32_t* npins = A.pins + pb;
-  for (int e = tid; e < n - 1; e += gsize) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  for (int e = tid; e < n - 1; e += gsize) emit_edge(A, bs, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
   // the net cost on a thread without an edge when there is one
   if (tid == (n - 1) % gsize) {
     A.out_edges[sb + n - 1] = -1;
@@ -1023,12 +1022,13 @@ struct GroupLds {
 };
 
 template <int G>
-__device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
+__device__ __forceinline__ void tree_group(const TreeArgs& A, int32_t bs, const int32_t* list, int n_list, int idx,
+                                           GroupLds& L) {
   const int lane = threadIdx.x & 63;
   const int gb = lane & ~(G - 1), tid = lane - gb;
   const unsigned long long gbits = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << gb);
-  const bool valid = idx < A.n_list;
-  const int pos = valid ? A.list[idx] : 0;
+  const bool valid = idx < n_list;
+  const int pos = valid ? list[idx] : 0;
   const int n = valid ? A.deg[pos] : 0;
   const int64_t pb = valid ? A.pinofs[pos] : 0;
   const uint32_t P = (uint32_t)(n * (n - 1) / 2);
@@ -1105,7 +1105,7 @@ __device__ void tree_group(const TreeArgs& A, int idx, GroupLds& L) {
   const int64_t sb = A.slot[pos];
   const int16_t* pat = A.pair_pat + A.pair[pos];
   const int32_t* npins = A.pins + pb;
-  for (int e = tid; e < n - 1; e += G) emit_edge(A, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  for (int e = tid; e < n - 1; e += G) emit_edge(A, bs, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
   if (tid == n - 1) {  // a lane without an edge
     A.out_edges[sb + n - 1] = -1;
     A.out_lpat[sb + n - 1] = -1;
@@ -1123,36 +1123,34 @@ struct SmallClasses {
 };
 
 template <int G>
-__device__ void run_group_class(const TreeArgs& A0, const SmallClasses& C, int cls, int b, int W, GroupLds* lds) {
-  TreeArgs A = A0;
-  A.list = C.list[cls];
-  A.n_list = C.n[cls];
+__device__ __forceinline__ void run_group_class(const TreeArgs& A, int32_t bs, const SmallClasses& C, int cls, int b,
+                                                int W, GroupLds* lds) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int idx = ((b - C.block0[cls]) * W + wv) * (64 / G) + lane / G;
-  tree_group<G>(A, idx, lds[wv]);
+  tree_group<G>(A, bs, C.list[cls], C.n[cls], idx, lds[wv]);
 }
 
 // Small-net work of block b (block0 ranges of C): W wavefronts, lds[W].
-__device__ void small_block(const TreeArgs& A, const SmallClasses& C, int b, int W, GroupLds* lds) {
+__device__ __forceinline__ void small_block(const TreeArgs& A, int32_t bs, const SmallClasses& C, int b, int W, GroupLds* lds) {
   if (b < C.block0[1]) {
     const int idx = b * 64 * W + threadIdx.x;
-    if (idx < C.n[0]) tree2_commit(A, C.list[0][idx]);
+    if (idx < C.n[0]) tree2_commit(A, bs, C.list[0][idx]);
   } else if (b < C.block0[2]) {
-    run_group_class<8>(A, C, 1, b, W, lds);
+    run_group_class<8>(A, bs, C, 1, b, W, lds);
   } else if (b < C.block0[3]) {
-    run_group_class<16>(A, C, 2, b, W, lds);
+    run_group_class<16>(A, bs, C, 2, b, W, lds);
   } else if (b < C.block0[4]) {
-    run_group_class<32>(A, C, 3, b, W, lds);
+    run_group_class<32>(A, bs, C, 3, b, W, lds);
   } else {
-    run_group_class<64>(A, C, 4, b, W, lds);
+    run_group_class<64>(A, bs, C, 4, b, W, lds);
   }
 }
 
 __global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C) {
-  TreeArgs A = A0;
-  A.bstamp = *A0.ctr;
+  const TreeArgs& A = A0;
+  const int32_t bs = *A0.ctr;  // this batch's stamp
   __shared__ GroupLds lds[4];
-  small_block(A, C, blockIdx.x, 4, lds);
+  small_block(A, bs, C, blockIdx.x, 4, lds);
 }
 
 // LDS carve-up of a workgroup tree for nets of up to cap pins, mcap matrix
@@ -1198,13 +1196,13 @@ __device__ inline TreeLds tree_lds(unsigned long long* shl, int cap, int mcap, i
 // largest net's keys do not fit in LDS); without them the code is leaner.
 template <bool MAT>
 __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int mcap, int scap) {
-  TreeArgs A = A0;
-  A.bstamp = *A0.ctr;
+  const TreeArgs& A = A0;
+  const int32_t bs = *A0.ctr;  // this batch's stamp
   extern __shared__ unsigned long long shl[];
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
   const TreeLds L = tree_lds(shl, cap, mcap, scap, wcnt, s_sh);
-  tree_commit<MAT>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
+  tree_commit<MAT>(A, bs, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
 }
 
 // One workgroup per net of more than kLargeDeg pins: the node arrays live in
@@ -1214,8 +1212,8 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int m
 // and the DSU hooking's compare-and-swap and volatile reads go to the L2.
 __global__ __launch_bounds__(1024) void k_tree_giant(TreeArgs A0, int mcap, uint8_t* scratch, int64_t nslots,
                                                      int64_t slot0) {
-  TreeArgs A = A0;
-  A.bstamp = *A0.ctr;
+  const TreeArgs& A = A0;
+  const int32_t bs = *A0.ctr;  // this batch's stamp
   extern __shared__ unsigned long long shl[];
   __shared__ int wcnt[((kMaxDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
@@ -1239,7 +1237,7 @@ __global__ __launch_bounds__(1024) void k_tree_giant(TreeArgs A0, int mcap, uint
   L.scap = 0;
   L.wcnt = wcnt;
   L.sh = s_sh;
-  tree_commit<true, true>(A, pos, threadIdx.x, blockDim.x, L);
+  tree_commit<true, true>(A, bs, pos, threadIdx.x, blockDim.x, L);
 }
 
 // Small and large nets of one batch in one launch (no second stream, so no
@@ -1251,17 +1249,17 @@ __global__ __launch_bounds__(1024) void k_tree_giant(TreeArgs A0, int mcap, uint
 template <bool MAT>
 __global__ __launch_bounds__(1024) void k_tree_mixed(TreeArgs A0, SmallClasses C, int n_large, int cap, int mcap,
                                                      int scap, int W) {
-  TreeArgs A = A0;
-  A.bstamp = *A0.ctr;
+  const TreeArgs& A = A0;
+  const int32_t bs = *A0.ctr;  // this batch's stamp
   extern __shared__ unsigned long long shl[];
   __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
   __shared__ int s_sh[2];
   if ((int)blockIdx.x < n_large) {
     const TreeLds L = tree_lds(shl, cap, mcap, scap, wcnt, s_sh);
-    tree_commit<MAT>(A, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
+    tree_commit<MAT>(A, bs, A.list[blockIdx.x], threadIdx.x, blockDim.x, L);
   } else {
     if ((int)(threadIdx.x >> 6) >= W) return;
-    small_block(A, C, (int)blockIdx.x - n_large, W, (GroupLds*)shl);
+    small_block(A, bs, C, (int)blockIdx.x - n_large, W, (GroupLds*)shl);
   }
 }
 

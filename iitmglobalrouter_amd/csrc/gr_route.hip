@@ -527,7 +527,6 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   A.mark = c->d_mark;
   A.run_stamp = c->d_run_stamp;
   A.ctr = c->d_ctr;
-  A.bstamp = 0;
   A.quirk = c->quirk;
   A.M = c->M;
   A.true_mst = c->true_mst;
