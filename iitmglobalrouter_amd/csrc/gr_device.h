@@ -441,24 +441,46 @@ __device__ __forceinline__ float wire_of(f32x2_a4 u, int pu, f32x2_a4 v, int pv)
 // straight one (whose moving wires end at d), replaces 8(Z-1) loads of single
 // terms; every term is formed from the same two prefix values by the same
 // fp32 subtraction, so the results are those of lane_best.
-template <int ZT>
+//
+// BUF: the pairs are read through a buffer resource whose byte 0 is the
+// padding float, so cell e's pair starts at byte 4e: the point's run offset
+// is a VGPR and the layer's offset l*XY*4 an SGPR (prefixes of fewer than
+// 2^30 cells).
+template <int ZT, bool BUF = false>
 __device__ __forceinline__ uint64_t lane_best_pts(const Geo& g, const float* __restrict__ pfx, int z1, int x1, int y1,
                                                   int z2, int x2, int y2) {
   const bool mx = x1 != x2, my = y1 != y2, D = mx && my;
   constexpr bool top_odd = ((ZT - 1) & 1) != 0;
   // the top layer carries no via term: its cells are needed for wires only
   const bool top_moves = top_odd ? my : mx;
+  __amdgpu_buffer_rsrc_t rs;
+  if constexpr (BUF)
+    rs = __builtin_amdgcn_make_buffer_rsrc((void*)(pfx - 1), (short)0, (int)((g.XY * (uint32_t)g.Z + 1u) * 4u),
+                                           0x00020000);
+  auto ld = [&](int l, int x, int y) -> f32x2_a4 {
+    if constexpr (BUF) {
+      typedef float f32x2 __attribute__((ext_vector_type(2)));
+      const uint32_t off = (l & 1) ? ((uint32_t)x * g.Y + (uint32_t)y) * 4u : ((uint32_t)y * g.X + (uint32_t)x) * 4u;
+      const f32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, (int)((uint32_t)l * g.XY * 4u), 0);
+      f32x2_a4 t;
+      t.x = v.x;
+      t.y = v.y;
+      return t;
+    } else {
+      return qpair(pfx, g, l, x, y);
+    }
+  };
   f32x2_a4 s[ZT], d[ZT], c1[ZT], c2[ZT];
 #pragma unroll
   for (int l = 0; l < ZT; ++l) {
-    if (l < ZT - 1 || top_moves) s[l] = qpair(pfx, g, l, x1, y1);
-    if (l < ZT - 1 || top_moves || D) d[l] = qpair(pfx, g, l, x2, y2);
+    if (l < ZT - 1 || top_moves) s[l] = ld(l, x1, y1);
+    if (l < ZT - 1 || top_moves || D) d[l] = ld(l, x2, y2);
   }
   if (D) {
 #pragma unroll
     for (int l = 0; l < ZT; ++l) {
-      c1[l] = qpair(pfx, g, l, x2, y1);
-      c2[l] = qpair(pfx, g, l, x1, y2);
+      c1[l] = ld(l, x2, y1);
+      c2[l] = ld(l, x1, y2);
     }
   }
   PairTerms<ZT> t;

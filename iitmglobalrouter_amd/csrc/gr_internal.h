@@ -43,6 +43,7 @@ struct BatchMeta {
   int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
   bool streams;           // the large nets' pair keys do not fit in the trees' LDS
   int32_t rowtask_ofs, n_rowtask;  // first-round row-minimum tasks (16 pins of a square-matrix net each)
+  int64_t pair_base;      // first pair of the batch in the schedule's pair order
 };
 
 struct Sched {
@@ -70,6 +71,14 @@ struct Sched {
   int64_t giant_bytes = 0;     // largest need over the batches with such nets
   int64_t sq_words = 0;        // largest square-matrix need over the batches
   int64_t rowmin_slots = 0;    // most output slots of a batch with square matrices
+  int64_t total_P = 0;         // pin pairs of all batches
+  // packed endpoints of every pin pair of the schedule (pair_base order,
+  // pack_cell layout), built once by build_sched for the resident schedule
+  // when the grid fits the packing; null: the costing kernel finds each
+  // pair's net and pins itself
+  uint2* d_ends = nullptr;
+  size_t cap_ends = 0;
+  bool ends_ok = false;
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;

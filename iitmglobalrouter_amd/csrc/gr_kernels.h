@@ -233,8 +233,9 @@ __device__ __forceinline__ void square_store(const SquareOut& sq, int pos, int j
   sq.key[so + k * n4 + j] = key;
 }
 
-// One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes).
-template <int ZT>
+// One lane per pin pair of the batch (workgroups of 64, 128 or 256 lanes);
+// BUF: prefix pairs through a buffer resource (prefixes of fewer than 2^30 cells).
+template <int ZT, bool BUF = true>
 __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restrict__ pfx,
                                                    const int32_t* __restrict__ pins,
                                                    const int64_t* __restrict__ pinofs,
@@ -253,10 +254,75 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   int z1, x1, y1, z2, x2, y2;
   dec(g, (uint32_t)pins[pb + j], z1, x1, y1);
   dec(g, (uint32_t)pins[pb + k], z2, x2, y2);
-  const uint64_t best = lane_best_pts<ZT>(g, pfx, z1, x1, y1, z2, x2, y2);
+  const uint64_t best = lane_best_pts<ZT, BUF>(g, pfx, z1, x1, y1, z2, x2, y2);
   out_cost[w] = key_cost((uint32_t)(best >> 32));
   out_pat[w] = (int16_t)(uint32_t)best;
   square_store(sq, own.pos, j, k, (uint32_t)(best >> 32));
+}
+
+// Packed pin cell of a pair endpoint table: z in bits 28..31, x in bits
+// 14..27, y in bits 0..13 (grids of at most 16,384 x 16,384 GCELLs and 16
+// layers).
+__host__ __device__ inline bool ends_fit(const Geo& g) { return g.X <= 16384 && g.Y <= 16384 && g.Z <= 16; }
+__device__ __forceinline__ uint32_t pack_cell(int z, int x, int y) {
+  return ((uint32_t)z << 28) | ((uint32_t)x << 14) | (uint32_t)y;
+}
+__device__ __forceinline__ void unpack_cell(uint32_t v, int& z, int& x, int& y) {
+  z = (int)(v >> 28);
+  x = (int)((v >> 14) & 0x3fffu);
+  y = (int)(v & 0x3fffu);
+}
+
+// Endpoint table of one batch: ends[w] = (pin j, pin k) of pair w, packed;
+// the nets and pins do not change between passes, so this is built once per
+// schedule and the costing kernel reads one coalesced 8-byte record per pair
+// instead of finding the pair's net and loading and decoding its two pins.
+__global__ __launch_bounds__(256) void k_pair_ends(Geo g, const int32_t* __restrict__ pins,
+                                                   const int64_t* __restrict__ pinofs,
+                                                   const int64_t* __restrict__ bpair,
+                                                   const int32_t* __restrict__ chunk_net, int n, int64_t P,
+                                                   uint2* __restrict__ ends) {
+  const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = w0 < P ? w0 : P - 1;
+  const PairOwner own = pair_owner_wave(bpair, pinofs, chunk_net, n, w);
+  if (w0 >= P) return;
+  int j, k;
+  pair_decode((uint32_t)(w - own.first_pair), j, k);
+  int z1, x1, y1, z2, x2, y2;
+  dec(g, (uint32_t)pins[own.first_pin + j], z1, x1, y1);
+  dec(g, (uint32_t)pins[own.first_pin + k], z2, x2, y2);
+  ends[w] = make_uint2(pack_cell(z1, x1, y1), pack_cell(z2, x2, y2));
+}
+
+// k_lpat_lane with the pair's endpoints from the schedule's endpoint table.
+// The pair's net is looked up only when the batch writes square key
+// matrices (sq.key set: the same for every lane of the launch).
+template <int ZT, bool BUF = true>
+__global__ __launch_bounds__(256) void k_lpat_ends(Geo g, const float* __restrict__ pfx,
+                                                   const uint2* __restrict__ ends,
+                                                   const int64_t* __restrict__ pinofs,
+                                                   const int64_t* __restrict__ bpair,
+                                                   const int32_t* __restrict__ chunk_net, int n, int64_t P,
+                                                   float* __restrict__ out_cost, int16_t* __restrict__ out_pat,
+                                                   int32_t* __restrict__ ctr, SquareOut sq) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;  // this batch's stamp, read by k_tree*/k_refresh
+  const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = w0 < P ? w0 : P - 1;  // tail lanes shadow the last pair until after the shuffles
+  const uint2 e = ends[w];
+  PairOwner own{0, 0, 0};
+  if (sq.key) own = pair_owner_wave(bpair, pinofs, chunk_net, n, w);
+  if (w0 >= P) return;
+  int z1, x1, y1, z2, x2, y2;
+  unpack_cell(e.x, z1, x1, y1);
+  unpack_cell(e.y, z2, x2, y2);
+  const uint64_t best = lane_best_pts<ZT, BUF>(g, pfx, z1, x1, y1, z2, x2, y2);
+  out_cost[w] = key_cost((uint32_t)(best >> 32));
+  out_pat[w] = (int16_t)(uint32_t)best;
+  if (sq.key) {
+    int j, k;
+    pair_decode((uint32_t)(w - own.first_pair), j, k);
+    square_store(sq, own.pos, j, k, (uint32_t)(best >> 32));
+  }
 }
 
 // Any Z (up to 32): one wavefront per pair, one lane per pattern.

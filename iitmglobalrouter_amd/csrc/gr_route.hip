@@ -168,7 +168,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
   h.chunk.clear();
   s.meta.assign(n_batches, BatchMeta{});
   s.giant_bytes = 0;
-  int64_t slots = 0, maxP = 0;
+  int64_t slots = 0, maxP = 0, total_P = 0;
   int32_t lo = 0;
   for (int b = 0; b < n_batches; ++b) {
     BatchMeta& m = s.meta[b];
@@ -194,6 +194,8 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
       P += d * (d - 1) / 2;
       runs += c->h_net_runs[id];
     }
+    m.pair_base = total_P;
+    total_P += P;
     m.runs = (int32_t)std::min<int64_t>(runs, (int64_t)c->g.Z * c->M);
     m.nslots = slots - m.slot0;
     for (int cls = 0; cls < kNumCls; ++cls) {
@@ -241,6 +243,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
   s.n_pos = npos;
   s.slots = slots;
   s.max_P = maxP;
+  s.total_P = total_P;
   h.slots = slots;
   h.maxP = maxP;
   return 0;
@@ -265,6 +268,28 @@ int upload_sched(gr_ctx* c, Sched& s, const HostSched& h) {
     HIPC(c, hipMemcpyAsync(s.d_chunk, h.chunk.data(), sizeof(int32_t) * h.chunk.size(), hipMemcpyHostToDevice,
                            c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));  // the host vectors may change after this
+  return 0;
+}
+
+// Endpoint table of the resident schedule (k_pair_ends per batch), for grids
+// that fit its packing; other schedules cost without one.
+int build_ends(gr_ctx* c, Sched& s) {
+  s.ends_ok = false;
+  if (&s != &c->sched || s.ext || !ends_fit(c->g) || s.total_P <= 0) return 0;
+  int rc;
+  if ((size_t)s.total_P > s.cap_ends) {
+    if ((rc = dalloc(c, &s.d_ends, (size_t)s.total_P))) return rc;
+    s.cap_ends = (size_t)s.total_P;
+  }
+  for (const BatchMeta& m : s.meta) {
+    if (m.P <= 0) continue;
+    hipLaunchKernelGGL(k_pair_ends, dim3((uint32_t)((m.P + 255) / 256)), dim3(256), 0, c->stream, c->g, c->d_pins,
+                       s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
+                       s.d_ends + m.pair_base);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(c->stream));
+  s.ends_ok = true;
   return 0;
 }
 
@@ -320,7 +345,8 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
       return rc;
     c->pair_cap = maxP;
   }
-  return upload_sched(c, s, h);
+  if ((rc = upload_sched(c, s, h))) return rc;
+  return build_ends(c, s);
 }
 
 typedef void (*LpatFn)(Geo, const float*, const int32_t*, const int64_t*, const int64_t*, const int32_t*, int,
@@ -355,20 +381,48 @@ PrefixFn prefix_kernel(int Z) {
   }
 }
 
-LpatFn lane_kernel(int Z) {
+template <bool BUF>
+LpatFn lane_kernel_for(int Z) {
   switch (Z) {
-    case 3: return k_lpat_lane<3>;
-    case 4: return k_lpat_lane<4>;
-    case 5: return k_lpat_lane<5>;
-    case 6: return k_lpat_lane<6>;
-    case 7: return k_lpat_lane<7>;
-    case 8: return k_lpat_lane<8>;
-    case 9: return k_lpat_lane<9>;
-    case 10: return k_lpat_lane<10>;
-    case 11: return k_lpat_lane<11>;
-    case 12: return k_lpat_lane<12>;
+    case 3: return k_lpat_lane<3, BUF>;
+    case 4: return k_lpat_lane<4, BUF>;
+    case 5: return k_lpat_lane<5, BUF>;
+    case 6: return k_lpat_lane<6, BUF>;
+    case 7: return k_lpat_lane<7, BUF>;
+    case 8: return k_lpat_lane<8, BUF>;
+    case 9: return k_lpat_lane<9, BUF>;
+    case 10: return k_lpat_lane<10, BUF>;
+    case 11: return k_lpat_lane<11, BUF>;
+    case 12: return k_lpat_lane<12, BUF>;
     default: return nullptr;
   }
+}
+typedef void (*EndsFn)(Geo, const float*, const uint2*, const int64_t*, const int64_t*, const int32_t*, int, int64_t,
+                       float*, int16_t*, int32_t*, SquareOut);
+template <bool BUF>
+EndsFn ends_kernel_for(int Z) {
+  switch (Z) {
+    case 3: return k_lpat_ends<3, BUF>;
+    case 4: return k_lpat_ends<4, BUF>;
+    case 5: return k_lpat_ends<5, BUF>;
+    case 6: return k_lpat_ends<6, BUF>;
+    case 7: return k_lpat_ends<7, BUF>;
+    case 8: return k_lpat_ends<8, BUF>;
+    case 9: return k_lpat_ends<9, BUF>;
+    case 10: return k_lpat_ends<10, BUF>;
+    case 11: return k_lpat_ends<11, BUF>;
+    case 12: return k_lpat_ends<12, BUF>;
+    default: return nullptr;
+  }
+}
+// Buffer-resource loads when the prefix (one padding float and a float per
+// cell) spans fewer than 2^31 bytes, flat loads otherwise.
+bool prefix_buf(const Geo& g) { return ((uint64_t)g.XY * (uint64_t)g.Z + 1) * 4 <= 0x7fffffffull; }
+LpatFn lane_kernel(const Geo& g) {
+  return prefix_buf(g) ? lane_kernel_for<true>(g.Z) : lane_kernel_for<false>(g.Z);
+}
+EndsFn ends_kernel(const Geo& g) {
+  return prefix_buf(g) ? ends_kernel_for<true>(g.Z) : ends_kernel_for<false>(g.Z);
 }
 
 // Tree stage of one batch.  Small nets (class lists 0..4) and large nets are
@@ -491,8 +545,14 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
     }
   } else if (m.P > 0) {
     KTimer t(c, GR_K_LPAT);
-    LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g.Z);
-    if (fn) {
+    LpatFn fn = c->tune.lpat_wave ? nullptr : lane_kernel(c->g);
+    EndsFn efn = (fn && s.ends_ok) ? ends_kernel(c->g) : nullptr;
+    if (efn) {
+      const int lb = lpat_block_for(c->tune.lpat_block, m.P);
+      hipLaunchKernelGGL(efn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
+                         s.d_ends + m.pair_base, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs,
+                         m.n, m.P, pcost, ppat, c->d_ctr, sq);
+    } else if (fn) {
       const int lb = lpat_block_for(c->tune.lpat_block, m.P);
       hipLaunchKernelGGL(fn, dim3((uint32_t)((m.P + lb - 1) / lb)), dim3(lb), 0, c->stream, c->g, c->d_pfx,
                          c->d_pins, s.d_pinofs + m.begin, s.d_pair + m.begin, s.d_chunk + m.chunk_ofs, m.n, m.P,
@@ -562,7 +622,7 @@ void free_sched(Sched& s) {
   (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
   (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq); (void)hipFree(s.d_rowtask);
-  (void)hipFree(s.d_rowmin);
+  (void)hipFree(s.d_rowmin); (void)hipFree(s.d_ends);
   s = Sched();
 }
 
@@ -1094,7 +1154,7 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
       for (int fk : {3, 1, 0})
         for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
-            if (lw && !lane_kernel(c->g.Z)) continue;  // only the wave kernel exists for this Z
+            if (lw && !lane_kernel(c->g)) continue;  // only the wave kernel exists for this Z
             if ((rc = time_one(gr_tuning{lt, gm, fk, rb, lw, 0}))) goto done;
           }
   // then the lane kernel's workgroup size, around the winner
