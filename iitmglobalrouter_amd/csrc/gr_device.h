@@ -33,6 +33,19 @@ __device__ __forceinline__ void dec(const Geo& g, uint32_t v, int& z, int& x, in
   y = (int)((z & 1) ? m : q);
 }
 
+// Packed pin cell of a pair endpoint table: z in bits 28..31, x in bits
+// 14..27, y in bits 0..13 (grids of at most 16,384 x 16,384 GCELLs and 16
+// layers).
+__host__ __device__ inline bool ends_fit(const Geo& g) { return g.X <= 16384 && g.Y <= 16384 && g.Z <= 16; }
+__device__ __forceinline__ uint32_t pack_cell(int z, int x, int y) {
+  return ((uint32_t)z << 28) | ((uint32_t)x << 14) | (uint32_t)y;
+}
+__device__ __forceinline__ void unpack_cell(uint32_t v, int& z, int& x, int& y) {
+  z = (int)(v >> 28);
+  x = (int)((v >> 14) & 0x3fffu);
+  y = (int)(v & 0x3fffu);
+}
+
 // e^x in double for cell costs of grids with fp32 capacities.  Cody-Waite
 // reduction x = k ln2 + t, |t| <= ln2/2 (W. J. Cody, W. Waite, Software Manual
 // for the Elementary Functions, 1980; ln2 split into its nearest double and the

@@ -260,19 +260,6 @@ __global__ __launch_bounds__(256) void k_lpat_lane(Geo g, const float* __restric
   square_store(sq, own.pos, j, k, (uint32_t)(best >> 32));
 }
 
-// Packed pin cell of a pair endpoint table: z in bits 28..31, x in bits
-// 14..27, y in bits 0..13 (grids of at most 16,384 x 16,384 GCELLs and 16
-// layers).
-__host__ __device__ inline bool ends_fit(const Geo& g) { return g.X <= 16384 && g.Y <= 16384 && g.Z <= 16; }
-__device__ __forceinline__ uint32_t pack_cell(int z, int x, int y) {
-  return ((uint32_t)z << 28) | ((uint32_t)x << 14) | (uint32_t)y;
-}
-__device__ __forceinline__ void unpack_cell(uint32_t v, int& z, int& x, int& y) {
-  z = (int)(v >> 28);
-  x = (int)((v >> 14) & 0x3fffu);
-  y = (int)(v & 0x3fffu);
-}
-
 // Endpoint table of one batch: ends[w] = (pin j, pin k) of pair w, packed;
 // the nets and pins do not change between passes, so this is built once per
 // schedule and the costing kernel reads one coalesced 8-byte record per pair

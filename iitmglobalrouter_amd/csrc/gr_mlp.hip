@@ -573,6 +573,7 @@ struct PropArgs {
   const int64_t* pinofs;
   const int64_t* bpair;
   const int32_t* chunk_net;
+  const uint2* ends;  // the batch's packed pair endpoints (pack_cell), or null: found from the nets
   int n;
   int64_t P;          // pairs of the batch, or input rows
   const uint16_t* X;  // input rows [P][k_in] (forward only)
@@ -622,13 +623,19 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     for (int i = tid; i < kLut; i += NT) slut[i] = f2bf(A.loglut[i]);
     if (tid < TT) {  // waves 0 and 1 whole: the owner search is wave-collective
       const int64_t w = t0 + (tid < nrow ? tid : nrow - 1);
-      const PairOwner own = pair_owner_wave(A.bpair, A.pinofs, A.chunk_net, A.n, w);
-      int j, k;
-      pair_decode((uint32_t)(w - own.first_pair), j, k);
-      const int64_t pb = own.first_pin;
       int z1, x1, y1, z2, x2, y2;
-      dec(g, (uint32_t)A.pins[pb + j], z1, x1, y1);
-      dec(g, (uint32_t)A.pins[pb + k], z2, x2, y2);
+      if (A.ends) {  // the same for every lane
+        const uint2 e = A.ends[w];
+        unpack_cell(e.x, z1, x1, y1);
+        unpack_cell(e.y, z2, x2, y2);
+      } else {
+        const PairOwner own = pair_owner_wave(A.bpair, A.pinofs, A.chunk_net, A.n, w);
+        int j, k;
+        pair_decode((uint32_t)(w - own.first_pair), j, k);
+        const int64_t pb = own.first_pin;
+        dec(g, (uint32_t)A.pins[pb + j], z1, x1, y1);
+        dec(g, (uint32_t)A.pins[pb + k], z2, x2, y2);
+      }
       sc[tid][0] = (int16_t)z1; sc[tid][1] = (int16_t)x1; sc[tid][2] = (int16_t)y1;
       sc[tid][3] = (int16_t)z2; sc[tid][4] = (int16_t)x2; sc[tid][5] = (int16_t)y2;
       const int dx = x2 - x1, dy = y2 - y1;
@@ -1038,6 +1045,7 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.pinofs = s.d_pinofs + bm.begin;
     A.bpair = s.d_pair + bm.begin;
     A.chunk_net = s.d_chunk + bm.chunk_ofs;
+    A.ends = s.ends_ok ? s.d_ends + bm.pair_base : nullptr;
     A.n = bm.n;
     A.P = bm.P;
     A.topk = c->prop_k;
