@@ -116,7 +116,7 @@ struct gr_ctx {
   int32_t *d_rem = nullptr, *d_cap0 = nullptr, *d_run_stamp = nullptr;
   uint8_t* d_mark = nullptr;  // per cell: marked by this batch's commit
   float *d_pfx = nullptr, *d_lut = nullptr;
-  float* d_pfx_mem = nullptr;  // the prefix's allocation: one padding float, then d_pfx
+  float* d_pfx_mem = nullptr;  // the prefix's allocation: four padding floats (zero), then d_pfx
   float* d_vterm = nullptr;  // non-BIT mode: per cell via cost of the layer, written by the refresh
   int32_t* d_ctr = nullptr;  // device batch counter: value = current batch stamp
   float *d_remf = nullptr, *d_capf0 = nullptr;  // fp32 grids: remaining and initial capacity

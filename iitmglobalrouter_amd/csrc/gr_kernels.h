@@ -8,7 +8,9 @@
 //   pfx[cell]        fp32   Q(pos) of the run's Fenwick tree, materialised per
 //                           cell, so a reference range query Q(hi) - Q(lo-1)
 //                           (bit_util.hpp:36-38) costs two loads
-//                           (one float of padding precedes the first cell)
+//                           (four floats of padding precede the first cell,
+//                           so a run of a multiple of 4 cells starts 16-byte
+//                           aligned when its offset is a multiple of 4)
 //   mark[cell]       uint8  1 from the commit that used the cell until the
 //                           same batch's refresh consumes it
 //   run_stamp[z*M+l] int32  batch stamp of the last commit that touched run l of layer z
@@ -52,9 +54,17 @@ namespace grd {
 // from the lane below), so only nodes of 512 cells and more need LDS
 // barriers.  Every node is (left child) + (right child) in fp32, as in a
 // level-by-level build, and a node exists only when its whole block lies in
-// the run.  The descent's node addresses depend only on x, so all its loads
-// are issued together; unset bits add +0.0, an exact identity on the
-// non-negative sums.
+// the run.  The loads of two 1024-cell chunks are issued together.
+//
+// Descent per quad: for x = 4t + q, Q(x) adds the nodes on the path of
+// i = x + 1 from its lowest set bit up (i -= lowbit(i)): the node of set bit
+// b of i is F[((i >> b) << b) - 1].  For q = 0..2 the path is the quad's own
+// nodes (F[4t] for q = 0, F[4t+1] for q = 1, F[4t+2] then F[4t+1] for q = 2)
+// followed by the path of 4t; for q = 3 it is the path of 4t + 4.  So a quad
+// reads the path of 4t once for three of its cells; an unset bit adds +0.0,
+// an exact identity on the non-negative sums.  A quad is written with one
+// 16-byte store when the run is aligned: values below the lowest marked cell
+// are rewritten with the bits they hold (their nodes cover unchanged cells).
 //
 // FLT: grids with fp32 capacities.  remf holds r per cell; a marked cell's r
 // is lowered by 1.0f in fp32 (bit_util.hpp:87, d_cube_data_bit[tid]--) and the
@@ -65,6 +75,11 @@ namespace grd {
 // marked cell's entry is rewritten with its new value.  VT: vterm holds the
 // per-cell via terms the proposal's candidate costing reads, rewritten with
 // the prefix (LDS then holds the run's prefix too).
+//
+// LDS: the cost table, then F and R, each M rounded up to a multiple of 4
+// floats (refresh_lds).
+__host__ __device__ inline int refresh_stride(int M) { return (M + 3) & ~3; }
+
 template <bool FLT, bool VT = false>
 __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  uint8_t* __restrict__ mark, float* __restrict__ pfx,
@@ -80,7 +95,7 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
   const int32_t bstamp = all ? 0 : *ctr;
   float* slut = sh;
   float* F = sh + kLut;
-  float* R = F + M;  // the run's prefix, for stored via terms (vterm non-null)
+  float* R = F + refresh_stride(M);  // the run's prefix, for stored via terms (vterm non-null)
   if (threadIdx.x < 2) first_hit[threadIdx.x] = 0x7fffffff;
   // the cost table goes to LDS at the first run the workgroup processes,
   // after that run's first cell loads are issued (a workgroup whose runs
@@ -98,18 +113,36 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     const uint32_t base = (uint32_t)z * g.XY + (uint32_t)line * L;
     if (threadIdx.x == 0) first_hit[slot ^ 1] = 0x7fffffff;  // read last by the previous run
     int my_first = all ? 0 : 0x7fffffff;
+    // a run that starts at a multiple of 4 cells is read in quads (one
+    // 16-byte load of the state, one 4-byte load of the marks per thread)
+    const bool quads = (base & 3u) == 0;
     for (int c0 = 0; c0 < L; c0 += 1024) {
       const int t4 = c0 + 4 * (int)threadIdx.x;
       float v[4];
       bool hits[4];
       int rv[4];
       float rfv[4];
+      if (quads && t4 + 3 < L) {
+        const uchar4 mk = *reinterpret_cast<const uchar4*>(mark + base + t4);
+        hits[0] = !all && mk.x;
+        hits[1] = !all && mk.y;
+        hits[2] = !all && mk.z;
+        hits[3] = !all && mk.w;
+        if constexpr (FLT) {
+          const float4 t = *reinterpret_cast<const float4*>(remf + base + t4);
+          rfv[0] = t.x; rfv[1] = t.y; rfv[2] = t.z; rfv[3] = t.w;
+        } else {
+          const int4 t = *reinterpret_cast<const int4*>(rem + base + t4);
+          rv[0] = t.x; rv[1] = t.y; rv[2] = t.z; rv[3] = t.w;
+        }
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = t4 + q;
-        hits[q] = i < L && !all && mark[base + i];
-        if constexpr (FLT) rfv[q] = i < L ? remf[base + i] : 0.0f;
-        else rv[q] = i < L ? rem[base + i] : 0;
+        for (int q = 0; q < 4; ++q) {
+          const int i = t4 + q;
+          hits[q] = i < L && !all && mark[base + i];
+          if constexpr (FLT) rfv[q] = i < L ? remf[base + i] : 0.0f;
+          else rv[q] = i < L ? rem[base + i] : 0;
+        }
       }
       if (!ready) {
         if (!FLT)
@@ -187,17 +220,39 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
     slot ^= 1;
     // with stored via terms, Q(f - 1) is formed again for the first of them
     const int f0 = (VT && f > 0) ? f - 1 : f;
-    for (int x = f0 + threadIdx.x; x < L; x += blockDim.x) {
-      const uint32_t i1 = (uint32_t)x + 1;
-      float res = 0.0f;
+    for (int t4 = 4 * (int)threadIdx.x; t4 < L; t4 += 4 * (int)blockDim.x) {
+      if (t4 + 3 < f0) continue;
+      const float4 o = *reinterpret_cast<const float4*>(F + t4);  // entries past L are never used
+      // runs are at most 8192 cells, so bits 2..13 cover every path
+      const uint32_t u = (uint32_t)t4, w = (uint32_t)t4 + 4u;
+      float hv[12], gv[12];
 #pragma unroll
-      for (int b = 0; b < 14; ++b) {
-        const uint32_t ib = (i1 >> b) << b;
-        const float nv = ((i1 >> b) & 1u) ? F[ib - 1] : 0.0f;
-        res += nv;
+      for (int b = 2; b < 14; ++b) {
+        hv[b - 2] = ((u >> b) & 1u) ? F[((u >> b) << b) - 1] : 0.0f;
+        gv[b - 2] = ((w >> b) & 1u) ? F[((w >> b) << b) - 1] : 0.0f;
       }
-      if (!VT || x >= f) pfx[base + x] = res;
-      if (VT) R[x] = res;
+      float q0 = o.x, q1 = o.y, q2 = o.z + o.y, q3 = 0.0f;
+#pragma unroll
+      for (int b = 0; b < 12; ++b) {
+        q0 += hv[b];
+        q1 += hv[b];
+        q2 += hv[b];
+        q3 += gv[b];
+      }
+      if (quads && t4 + 3 < L) {
+        *reinterpret_cast<float4*>(pfx + base + t4) = make_float4(q0, q1, q2, q3);
+      } else {
+        const float qv[4] = {q0, q1, q2, q3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (t4 + q >= f0 && t4 + q < L) pfx[base + t4 + q] = qv[q];
+      }
+      if (VT) {
+        const float qv[4] = {q0, q1, q2, q3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (t4 + q < L) R[t4 + q] = qv[q];
+      }
     }
     if constexpr (VT) {
       // Q(pos) - Q(pos - 1), Q(-1) = 0 (bit_util.hpp:108-117): the fp32

@@ -107,7 +107,7 @@ struct KTimer {
 };
 
 size_t refresh_lds(const gr_ctx* c, bool with_vterm) {
-  return (kLut + (with_vterm ? 2 : 1) * (size_t)c->M) * sizeof(float);
+  return (kLut + (with_vterm ? 2 : 1) * (size_t)refresh_stride(c->M)) * sizeof(float);
 }
 size_t refresh_prefix_lds(const gr_ctx* c) { return (3 * (size_t)c->M + 128) * sizeof(float); }
 // Refresh grid: the tuned size, or fewer workgroups when the batch can touch
@@ -697,14 +697,14 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
     return bail(fail(nullptr, GR_ENODEV, "stream"));
   const size_t nr = (size_t)d->Z * c->M;
   if ((rc = dalloc(c, &c->d_rem, cs)) || (rc = dalloc(c, &c->d_cap0, cs)) || (rc = dalloc(c, &c->d_mark, cs)) ||
-      (rc = dalloc(c, &c->d_pfx_mem, cs + 1)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
+      (rc = dalloc(c, &c->d_pfx_mem, cs + 4)) || (rc = dalloc(c, &c->d_lut, kLut)) || (rc = dalloc(c, &c->d_run_stamp, nr)) ||
       (rc = dalloc(c, &c->d_ctr, 1)) ||
       ((capf || c->prefix) && ((rc = dalloc(c, &c->d_remf, cs)) || (rc = dalloc(c, &c->d_capf0, cs)))) ||
       (c->prefix && ((rc = dalloc(c, &c->d_vhalf, cs)) || (rc = dalloc(c, &c->d_vterm, cs))))) {
     g_err = c->err;
     return bail(rc);
   }
-  c->d_pfx = c->d_pfx_mem + 1;
+  c->d_pfx = c->d_pfx_mem + 4;  // 16-byte aligned: the refresh writes aligned quads
   c->lds_max = prop.sharedMemPerBlock;
   bool ok = hipFuncSetAttribute((const void*)k_refresh<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)refresh_lds(c, false)) == hipSuccess &&
@@ -745,7 +745,7 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
-      hipMemset(c->d_pfx_mem, 0, sizeof(float)) != hipSuccess ||
+      hipMemset(c->d_pfx_mem, 0, 4 * sizeof(float)) != hipSuccess ||
       (c->d_vhalf && hipMemset(c->d_vhalf, 0, sizeof(int32_t) * cs) != hipSuccess) ||
       hipMemcpy(c->d_lut, c->lut, sizeof(float) * kLut, hipMemcpyHostToDevice) != hipSuccess) {
     g_err = "device set-up failed";
