@@ -303,7 +303,11 @@ __device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, 
     }
   };
 
-  uint64_t best = ~0ull;
+  // running minimum: strict-less on the cost in ascending p keeps the lowest
+  // p among equal costs, the order of (ord_key(cost), p) (costs are finite
+  // and not NaN; -0.0 and +0.0 compare equal in both)
+  float bc = __builtin_inff();
+  int bpat = -1;
   float vs = 0.f;  // running source via sum for a > z1
 #pragma unroll
   for (int a = 1; a < ZT; ++a) {
@@ -331,8 +335,9 @@ __device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, 
         for (int i = lo; i < hi; ++i) c += (a & 1) ? t.dc2[i] : t.dc1[i];
         c += t.w2[bp];
         to_dst(c, bp);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b);
-        best = key < best ? key : best;
+        const bool take = c < bc;
+        bc = take ? c : bc;
+        bpat = take ? a * Z2T + b : bpat;
       }
     } else {
       // straight pair: every b gives the same route, b = 0 is the lowest p
@@ -345,12 +350,13 @@ __device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, 
       if (at_dst && b0 >= 0) {
         float c = ca;
         to_dst(c, a);
-        const uint64_t key = ((uint64_t)ord_key(c) << 32) | (uint32_t)(a * Z2T + b0);
-        best = key < best ? key : best;
+        const bool take = c < bc;
+        bc = take ? c : bc;
+        bpat = take ? a * Z2T + b0 : bpat;
       }
     }
   }
-  return best;
+  return bpat < 0 ? ~0ull : (((uint64_t)ord_key(bc) << 32) | (uint32_t)bpat);
 }
 
 // vt: the via terms stored per cell (the refresh writes them while a learned
