@@ -294,12 +294,16 @@ __device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, 
     ddn[i] = i >= z2 ? t.dd[i] : 0.f;
     dup[i] = i < z2 ? t.dd[i] : 0.f;
   }
+  // (fixed trip counts with conditions on i, so that every loop unrolls
+  // fully once zs is known: no register-indexed loops)
   auto to_dst = [&](float& c, const int zs) {
 #pragma unroll
-    for (int i = 0; i < zs; ++i) c += ddn[i];
+    for (int i = 0; i < ZT - 1; ++i)
+      if (i < zs) c += ddn[i];
     if (z2 > zs) {
 #pragma unroll
-      for (int i = zs; i < ZT - 1; ++i) c += dup[i];
+      for (int i = 0; i < ZT - 1; ++i)
+        if (i >= zs) c += dup[i];
     }
   };
 
@@ -332,7 +336,8 @@ __device__ __forceinline__ uint64_t lane_select(const PairTerms<ZT>& t, int z1, 
         float c = ca;
         const int lo = a < bp ? a : bp, hi = a < bp ? bp : a;
 #pragma unroll
-        for (int i = lo; i < hi; ++i) c += (a & 1) ? t.dc2[i] : t.dc1[i];
+        for (int i = 0; i < ZT - 1; ++i)
+          if (i >= lo && i < hi) c += (a & 1) ? t.dc2[i] : t.dc1[i];
         c += t.w2[bp];
         to_dst(c, bp);
         const bool take = c < bc;
