@@ -830,8 +830,37 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
   const uint32_t* sqk = (MAT && !staged && A.sq && A.sqofs[pos] >= 0) ? A.sq + A.sqofs[pos] : nullptr;
   const int n4 = (int)square_stride(n);
   uint32_t* skey = L.skey;
-  if (staged) {
+  if (MAT && staged) {
+    // launches with matrix rounds are those whose keys stream: nothing to stage
     for (uint32_t o = tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
+  } else if (staged) {
+    // 16-byte loads from the first 16-byte boundary on, four per thread in
+    // flight before any is converted; the head and tail singly
+    const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(gcost) >> 2) & 3u;
+    const uint32_t head = min(P, (4u - mis) & 3u);
+    const uint32_t n4 = (P - head) >> 2;
+    const float4* g4 = reinterpret_cast<const float4*>(gcost + head);
+    for (uint32_t o = tid; o < head; o += gsize) skey[o] = ord_key(gcost[o]);
+    for (uint32_t q0 = tid; q0 < n4; q0 += 4u * gsize) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + (uint32_t)u * gsize;
+        if (q < n4) v[u] = g4[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + (uint32_t)u * gsize;
+        if (q < n4) {
+          uint32_t* d = skey + head + 4u * q;
+          d[0] = ord_key(v[u].x);
+          d[1] = ord_key(v[u].y);
+          d[2] = ord_key(v[u].z);
+          d[3] = ord_key(v[u].w);
+        }
+      }
+    }
+    for (uint32_t o = head + 4u * n4 + tid; o < P; o += gsize) skey[o] = ord_key(gcost[o]);
   }
   for (int i = tid; i < n; i += gsize) {
     root[i] = i;
@@ -1149,7 +1178,19 @@ __device__ __forceinline__ void tree_group(const TreeArgs& A, int32_t bs, const 
   uint32_t* key = L.key + (gb / G) * (G * (G - 1) / 2);
   if (valid) {
     const float* gcost = A.pair_cost + A.pair[pos];
-    for (uint32_t o = tid; o < P; o += G) key[o] = ord_key(gcost[o]);
+    for (uint32_t o0 = tid; o0 < P; o0 += 4 * G) {  // four loads in flight per lane
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t o = o0 + (uint32_t)(u * G);
+        v[u] = o < P ? gcost[o] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t o = o0 + (uint32_t)(u * G);
+        if (o < P) key[o] = ord_key(v[u]);
+      }
+    }
   }
   if (tid < n) root[tid] = tid;
   int ne = 0;
