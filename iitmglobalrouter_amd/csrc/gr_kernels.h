@@ -1020,6 +1020,10 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
         pr = (root[j] == r) ? root[k] : root[j];
       }
       partner[r] = pr;
+      if (!MAT) {
+        par[r] = r;       // this round's forest pointers
+        par_next[r] = r;  // the largest old root of each tree, below
+      }
     }
     tree_sync<GN>();
     GR_TREE_MARK(3);
@@ -1042,7 +1046,8 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
         off += __popcll(m & ((1ull << lane) - 1ull));
         edges[off] = recorded_edge(A, root, r, pr, (uint32_t)best[r]);
         ecost[off] = key_cost((uint32_t)(best[r] >> 32));  // pair costs are never -0.0
-        dsu_union(par, r, pr, 4 * n);
+        if (MAT) dsu_union(par, r, pr, 4 * n);
+        else par[r] = pr;
       }
     }
     tree_sync<GN>();
@@ -1050,11 +1055,26 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
     int tot = ne;
     for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
     if (tid == 0) sh[0] = tot;
-    // flatten: every node's root becomes its component's largest member
-    for (int i = tid; i < n; i += gsize) {
-      root[i] = dsu_find(par, root[i]);
-      best[i] = ~0ull;
-      par_next[i] = i;
+    if (MAT) {
+      // flatten: every node's root becomes its component's largest member
+      for (int i = tid; i < n; i += gsize) {
+        root[i] = dsu_find(par, root[i]);
+        best[i] = ~0ull;
+        par_next[i] = i;
+      }
+    } else {
+      // the accepted edges (root -> partner root) form a forest, one
+      // pointer per root, written without compare-and-swap; every old root
+      // finds its tree's root (path halving), and the tree's largest old root,
+      // i.e. its largest member, becomes the component's root, as hooking
+      // the smaller root under the larger makes it
+      for (int r = tid; r < n; r += gsize)
+        if (root[r] == r) atomicMax(&par_next[dsu_find(par, r)], r);
+      tree_sync<GN>();
+      for (int i = tid; i < n; i += gsize) {
+        root[i] = par_next[dsu_find(par, root[i])];
+        best[i] = ~0ull;
+      }
     }
     tree_sync<GN>();
     GR_TREE_MARK(5);
