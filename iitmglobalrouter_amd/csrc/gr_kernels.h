@@ -622,9 +622,9 @@ __device__ __forceinline__ void tree2_commit(const TreeArgs& A, int32_t bs, int 
   A.out_cost[A.cost_base + pos] = 0.0f + A.pair_cost[A.pair[pos]];
 }
 
-// Root of x with path halving: each visited non-root is re-pointed to its
-// grandparent, an ancestor, so concurrent finds and hookings (which only
-// change roots) stay valid and chains built by one round's unions shrink.
+// Root of x in a round's merge forest with path halving: each visited
+// non-root is re-pointed to its grandparent, an ancestor, so concurrent finds
+// stay valid and the chains shrink as they run.
 __device__ __forceinline__ int dsu_find(volatile int* vp, int x) {
   while (true) {
     const int p = vp[x];
@@ -632,21 +632,6 @@ __device__ __forceinline__ int dsu_find(volatile int* vp, int x) {
     const int pp = vp[p];
     if (pp != p) vp[x] = pp;
     x = pp;
-  }
-}
-
-// Lock-free DSU hooking on LDS: the smaller root is always hooked under the
-// larger (as gmst_util.hpp:314-325 does serially), so every component's root
-// stays its largest member whatever the order of concurrent unions.  A failed
-// CAS means another union moved a; retrying from the new roots always ends.
-__device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
-  volatile int* vp = par;
-  for (int guard = 0; guard < bound; ++guard) {
-    a = dsu_find(vp, a);
-    b = dsu_find(vp, b);
-    if (a == b) return;
-    if (a > b) { const int t = a; a = b; b = t; }
-    if (atomicCAS(&par[a], a, b) == a) return;
   }
 }
 
@@ -661,8 +646,12 @@ __device__ __forceinline__ void dsu_union(int* par, int a, int b, int bound) {
 //                mutual pair is dropped: the serial outcome, computed in
 //                parallel.  Accepted edges (root, partner) are written in
 //                ascending root order.
-//   merge        accepted edges are united by lock-free hooking; roots are
-//                then flattened (root = the component's largest member).
+//   merge        the accepted edges form a forest (each root has at most one,
+//                to its partner root): one plain pointer store per root, then
+//                each tree's largest old root, i.e. its largest member, is
+//                the new root of all its members (a find with path halving
+//                and an atomic maximum), which is the root that hooking the
+//                smaller root under the larger gives (gmst_util.hpp:314-325).
 // Rounds repeat until deg-1 edges (the reference's feedback loop,
 // mst_util.hpp:298-314, ends in the same state).
 //
@@ -708,7 +697,7 @@ struct TreeLds {
 // Workgroup barrier of the tree; with the node arrays in global memory (GN)
 // it is also an agent-scope release and acquire, so stores are in the L2
 // before the barrier and the L1 holds no stale line after it (the atomics of
-// the min phase and the DSU hooking act in the L2).
+// the min phase and the merge act in the L2).
 template <bool GN>
 __device__ __forceinline__ void tree_sync() {
   if (GN) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1020,10 +1009,8 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
         pr = (root[j] == r) ? root[k] : root[j];
       }
       partner[r] = pr;
-      if (!MAT) {
-        par[r] = r;       // this round's forest pointers
-        par_next[r] = r;  // the largest old root of each tree, below
-      }
+      par[r] = r;       // this round's forest pointers
+      par_next[r] = r;  // the largest old root of each tree, below
     }
     tree_sync<GN>();
     GR_TREE_MARK(3);
@@ -1046,8 +1033,7 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
         off += __popcll(m & ((1ull << lane) - 1ull));
         edges[off] = recorded_edge(A, root, r, pr, (uint32_t)best[r]);
         ecost[off] = key_cost((uint32_t)(best[r] >> 32));  // pair costs are never -0.0
-        if (MAT) dsu_union(par, r, pr, 4 * n);
-        else par[r] = pr;
+        par[r] = pr;
       }
     }
     tree_sync<GN>();
@@ -1055,14 +1041,7 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
     int tot = ne;
     for (int q = 0; q < iters * nwv; ++q) tot += wcnt[q];
     if (tid == 0) sh[0] = tot;
-    if (MAT) {
-      // flatten: every node's root becomes its component's largest member
-      for (int i = tid; i < n; i += gsize) {
-        root[i] = dsu_find(par, root[i]);
-        best[i] = ~0ull;
-        par_next[i] = i;
-      }
-    } else {
+    {
       // the accepted edges (root -> partner root) form a forest, one
       // pointer per root, written without compare-and-swap; every old root
       // finds its tree's root (path halving), and the tree's largest old root,
@@ -1263,11 +1242,17 @@ __device__ __forceinline__ void tree_group(const TreeArgs& A, int32_t bs, const 
       const int off = ne + __popcll(gm & ((1ull << lane) - 1ull));
       edges[off] = recorded_edge(A, root, tid, pr, (uint32_t)best[tid]);
       ecost[off] = key_cost((uint32_t)(best[tid] >> 32));  // pair costs are never -0.0
-      dsu_union(link, tid, pr, 4 * G);
+      link[tid] = pr;  // the accepted edges form a forest: one pointer per root
     }
     ne += __popcll(gm);
     wave_sync();
-    if (me) root[tid] = dsu_find(link, root[tid]);
+    // each tree's largest old root (its largest member) becomes the
+    // component's root; partner is free after the accept and holds the maxima
+    if (me) partner[tid] = tid;
+    wave_sync();
+    if (me && root[tid] == tid) atomicMax(&partner[dsu_find(link, tid)], tid);
+    wave_sync();
+    if (me) root[tid] = partner[dsu_find(link, root[tid])];
     wave_sync();
   }
   if (!valid) return;
@@ -1378,7 +1363,7 @@ __global__ __launch_bounds__(1024) void k_tree_large(TreeArgs A0, int cap, int m
 // global memory (44 B per slot of the batch, at the net's slot offset), the
 // LDS holds the component matrix; keys are always streamed.  Global memory
 // written by the workgroup's own threads is seen by them after each barrier,
-// and the DSU hooking's compare-and-swap and volatile reads go to the L2.
+// and the merge's atomic maxima and volatile reads go to the L2.
 __global__ __launch_bounds__(1024) void k_tree_giant(TreeArgs A0, int mcap, uint8_t* scratch, int64_t nslots,
                                                      int64_t slot0) {
   const TreeArgs& A = A0;
