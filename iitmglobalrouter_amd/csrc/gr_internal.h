@@ -41,6 +41,8 @@ struct BatchMeta {
   int64_t max_large_P;    // most pin pairs of a large net in the batch
   int32_t max_giant;      // most pins of a net above kLargeDeg in the batch (0: none)
   int32_t runs;           // bound on the runs the batch's commit can touch (refresh grid)
+  int32_t rlist_ofs = -1; // first entry of the batch's candidate-run list (-1: no list, every run)
+  int32_t nrl = 0;        // entries of that list
   bool streams;           // the large nets' pair keys do not fit in the trees' LDS
   int32_t rowtask_ofs, n_rowtask;  // first-round row-minimum tasks (16 pins of a square-matrix net each)
   int64_t pair_base;      // first pair of the batch in the schedule's pair order
@@ -67,7 +69,9 @@ struct Sched {
   uint32_t* d_sq = nullptr;
   int2* d_rowtask = nullptr;          // (batch position, first pin) of each row-minimum task
   unsigned long long* d_rowmin = nullptr;  // per output slot: the pin's first-round minimum (key << 32 | pair)
+  int32_t* d_rlist = nullptr;  // candidate-run lists of the batches (BatchMeta::rlist_ofs)
   size_t cap_pos = 0, cap_slots = 0, cap_chunk = 0, cap_giant = 0, cap_sq = 0, cap_rowtask = 0, cap_rowmin = 0;
+  size_t cap_rlist = 0;
   int64_t giant_bytes = 0;     // largest need over the batches with such nets
   int64_t sq_words = 0;        // largest square-matrix need over the batches
   int64_t rowmin_slots = 0;    // most output slots of a batch with square matrices
@@ -94,6 +98,7 @@ struct HostSched {
   std::vector<int64_t> slot, pair, pinofs, sqofs;
   std::vector<int32_t> degv, list, chunk;
   std::vector<int2> rowtask;
+  std::vector<int32_t> rlist;  // per batch: the runs its commit can stamp, when fewer than all
   int64_t slots = 0, maxP = 0;
 };
 
@@ -129,6 +134,7 @@ struct gr_ctx {
   int32_t n_nets = 0;
   std::vector<int64_t> h_pin_pfx;
   std::vector<int32_t> h_net_runs;  // per net: bound on the runs its routes can touch
+  std::vector<int32_t> h_net_box;   // per net: x0, x1, y0, y1 of its pins' bounding box
   int64_t* d_pin_pfx = nullptr;
   int32_t* d_pins = nullptr;
   grh::Sched sched, single;

@@ -37,7 +37,8 @@ namespace grd {
 // Cost refresh
 // ---------------------------------------------------------------------------
 
-// Runs are scanned by a persistent grid; run id r = z*M + line.  all != 0
+// Runs are scanned by a persistent grid; run id r = z*M + line, every run or
+// the batch's candidate list rl[0, nrl) (runs its commit can stamp).  all != 0
 // rebuilds every run without touching demand (set-up and reset).  Otherwise a
 // run is processed only when the batch's commit stamped it: its marked cells
 // lose one unit of capacity (bit_util.hpp:75-90 applies cap -= 1 once per
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const int32_t* __restrict__ run_stamp,
                                                  const int32_t* __restrict__ ctr, int all,
                                                  float* __restrict__ remf, int model,
-                                                 int16_t* __restrict__ remz, float* __restrict__ vterm) {
+                                                 int16_t* __restrict__ remz, float* __restrict__ vterm,
+                                                 const int32_t* __restrict__ rl, int nrl) {
   extern __shared__ float sh[];
   // lowest marked position of the run, in two slots that alternate between
   // the runs a workgroup processes: a slot is cleared one run before its use
@@ -103,8 +105,9 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
   bool ready = false;
   int slot = 0;
   const int lane = threadIdx.x & 63;
-  const int nr = g.Z * M;
-  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+  const int nr = rl ? nrl : g.Z * M;
+  for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
+    const int r = rl ? rl[ri] : ri;
     const int z = r / M, line = r - z * M;
     const int lines = (z & 1) ? g.X : g.Y;
     if (line >= lines) continue;

@@ -40,7 +40,8 @@ __global__ __launch_bounds__(256) void k_refresh_prefix(Geo g, int M, float* __r
                                                         uint8_t* __restrict__ mark, int32_t* __restrict__ vhalf,
                                                         float* __restrict__ pfx, float* __restrict__ vterm,
                                                         const int32_t* __restrict__ run_stamp,
-                                                        const int32_t* __restrict__ ctr, int all) {
+                                                        const int32_t* __restrict__ ctr, int all,
+                                                        const int32_t* __restrict__ rl, int nrl) {
   extern __shared__ float sh[];
   float* A = sh;
   float* W = A + M;
@@ -48,8 +49,9 @@ __global__ __launch_bounds__(256) void k_refresh_prefix(Geo g, int M, float* __r
   float* C = H + M;  // 128 chunk-end prefixes
   const int32_t bstamp = all ? 0 : *ctr;
   const int T = (M + 127) / 128;
-  const int nr = g.Z * M;
-  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+  const int nr = rl ? nrl : g.Z * M;  // every run, or the batch's candidate list
+  for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
+    const int r = rl ? rl[ri] : ri;
     const int z = r / M, line = r - z * M;
     const int lines = (z & 1) ? g.X : g.Y;
     if (line >= lines) continue;

@@ -119,20 +119,24 @@ int refresh_grid(const gr_ctx* c, int runs) {
   return std::max(1, g);
 }
 
-void launch_refresh(gr_ctx* c, int all, int runs = -1) {
+// rl (nrl entries): the batch's candidate runs, one workgroup each up to the
+// tuned grid; null: every run, with the grid bounded by runs.
+void launch_refresh(gr_ctx* c, int all, int runs = -1, const int32_t* rl = nullptr, int nrl = 0) {
+  const int grid = rl ? std::max(1, std::min(nrl, c->tune.refresh_blocks)) : refresh_grid(c, runs);
   if (c->prefix)
-    hipLaunchKernelGGL(k_refresh_prefix, dim3(refresh_grid(c, runs)), dim3(256), refresh_prefix_lds(c), c->stream,
+    hipLaunchKernelGGL(k_refresh_prefix, dim3(grid), dim3(256), refresh_prefix_lds(c), c->stream,
                        c->g, c->M, c->d_remf, c->d_mark, c->d_vhalf, c->d_pfx, c->d_vterm, c->d_run_stamp, c->d_ctr,
-                       all);
+                       all, rl, nrl);
   else if (c->d_remf)
-    hipLaunchKernelGGL(k_refresh<true>, dim3(refresh_grid(c, runs)), dim3(256), refresh_lds(c, false), c->stream,
+    hipLaunchKernelGGL(k_refresh<true>, dim3(grid), dim3(256), refresh_lds(c, false), c->stream,
                        c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf,
-                       c->model, (int16_t*)nullptr, (float*)nullptr);
+                       c->model, (int16_t*)nullptr, (float*)nullptr, rl, nrl);
   else
-    hipLaunchKernelGGL((c->vterm_live ? k_refresh<false, true> : k_refresh<false, false>), dim3(refresh_grid(c, runs)),
+    hipLaunchKernelGGL((c->vterm_live ? k_refresh<false, true> : k_refresh<false, false>), dim3(grid),
                        dim3(256), refresh_lds(c, c->vterm_live), c->stream, c->g, c->M, c->d_rem, c->d_mark, c->d_pfx,
                        c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
-                       c->remz_live ? c->d_remz : (int16_t*)nullptr, c->vterm_live ? c->d_vterm : (float*)nullptr);
+                       c->remz_live ? c->d_remz : (int16_t*)nullptr, c->vterm_live ? c->d_vterm : (float*)nullptr,
+                       rl, nrl);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -153,9 +157,48 @@ bool tree_streams(const gr_ctx* c, const BatchMeta& m) {
   return (int64_t)tree_lds_bytes(m.max_large, 0, (int)std::min<int64_t>(m.max_large_P, 1 << 30)) > budget;
 }
 
+// Candidate runs of one batch (appended to h.rlist, m.rlist_ofs / m.nrl set)
+// when they are fewer than three quarters of all runs: the rows of each net's
+// box on even layers and its columns on odd ones, each range with one line
+// more below for the quirk cells just before a first wire's low end (they
+// fall in the previous run of the layer), and the last run of every layer
+// when a box touches line 0 (the previous run of line 0 is the previous
+// layer's last one).  Grids with a side of at most Z + 1 cells, where quirk
+// cells could reach further back, refresh every run.
+void batch_runs(const gr_ctx* c, HostSched& h, BatchMeta& m, const int32_t* ids) {
+  m.rlist_ofs = -1;
+  m.nrl = 0;
+  const Geo& g = c->g;
+  const int total = g.Z * c->M;
+  if ((int)c->h_net_box.size() < 4 * c->n_nets || std::min(g.X, g.Y) <= g.Z + 1 || m.runs >= total) return;
+  std::vector<uint8_t> rows(g.Y, 0), cols(g.X, 0);
+  bool wrap = false;
+  for (int i = 0; i < m.n; ++i) {
+    const int32_t* b = &c->h_net_box[4 * (size_t)ids[i]];
+    for (int y = std::max(b[2] - 1, 0); y <= b[3]; ++y) rows[y] = 1;
+    for (int x = std::max(b[0] - 1, 0); x <= b[1]; ++x) cols[x] = 1;
+    wrap = wrap || b[0] == 0 || b[2] == 0;
+  }
+  const size_t at = h.rlist.size();
+  for (int z = 0; z < g.Z; ++z) {
+    const int lines = (z & 1) ? g.X : g.Y;
+    const uint8_t* cov = (z & 1) ? cols.data() : rows.data();
+    for (int l = 0; l < lines; ++l)
+      if (cov[l] || (wrap && l == lines - 1)) h.rlist.push_back(z * c->M + l);
+  }
+  const size_t n = h.rlist.size() - at;
+  if (4 * n >= 3 * (size_t)total) {
+    h.rlist.resize(at);
+    return;
+  }
+  m.rlist_ofs = (int32_t)at;
+  m.nrl = (int32_t)n;
+}
+
 int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32_t* batch_ofst,
-               const int32_t* net_ids) {
+               const int32_t* net_ids, bool lists = false) {
   const int32_t npos = batch_ofst[n_batches];
+  h.rlist.clear();
   h.slot.assign(npos, 0);
   h.sqofs.assign(npos, -1);
   h.rowtask.clear();
@@ -197,6 +240,7 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
     m.pair_base = total_P;
     total_P += P;
     m.runs = (int32_t)std::min<int64_t>(runs, (int64_t)c->g.Z * c->M);
+    if (lists) batch_runs(c, h, m, net_ids + m.begin);
     m.nslots = slots - m.slot0;
     for (int cls = 0; cls < kNumCls; ++cls) {
       m.cls_ofs[cls] = lo;
@@ -267,6 +311,9 @@ int upload_sched(gr_ctx* c, Sched& s, const HostSched& h) {
   if (!h.chunk.empty())
     HIPC(c, hipMemcpyAsync(s.d_chunk, h.chunk.data(), sizeof(int32_t) * h.chunk.size(), hipMemcpyHostToDevice,
                            c->stream));
+  if (s.d_rlist && !h.rlist.empty())
+    HIPC(c, hipMemcpyAsync(s.d_rlist, h.rlist.data(), sizeof(int32_t) * h.rlist.size(), hipMemcpyHostToDevice,
+                           c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));  // the host vectors may change after this
   return 0;
 }
@@ -300,8 +347,12 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   // boundaries and device pointers: rebuilding that schedule invalidates it.
   if (&s == &c->sched) drop_graph(c);
   HostSched h;
-  int rc = host_sched(c, s, h, n_batches, batch_ofst, net_ids);
+  int rc = host_sched(c, s, h, n_batches, batch_ofst, net_ids, true);
   if (rc) return rc;
+  if (h.rlist.size() > s.cap_rlist) {
+    if ((rc = dalloc(c, &s.d_rlist, h.rlist.size()))) return rc;
+    s.cap_rlist = h.rlist.size();
+  }
   const size_t npos = h.slot.size();
   const int64_t slots = h.slots, maxP = h.maxP;
   if (npos > s.cap_pos) {
@@ -608,7 +659,8 @@ int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   }
   {
     KTimer t(c, GR_K_REFRESH);
-    launch_refresh(c, 0, m.runs);
+    if (m.rlist_ofs >= 0 && s.d_rlist) launch_refresh(c, 0, m.runs, s.d_rlist + m.rlist_ofs, m.nrl);
+    else launch_refresh(c, 0, m.runs);
   }
   HIPC(c, hipGetLastError());
   c->last_P = m.P;
@@ -622,7 +674,7 @@ void free_sched(Sched& s) {
   (void)hipFree(s.d_pinofs); (void)hipFree(s.d_deg); (void)hipFree(s.d_slot); (void)hipFree(s.d_pair); (void)hipFree(s.d_list);
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
   (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq); (void)hipFree(s.d_rowtask);
-  (void)hipFree(s.d_rowmin); (void)hipFree(s.d_ends);
+  (void)hipFree(s.d_rowmin); (void)hipFree(s.d_ends); (void)hipFree(s.d_rlist);
   s = Sched();
 }
 
@@ -821,6 +873,7 @@ int gr_set_nets(gr_ctx* c, int32_t n_nets, const int64_t* pin_pfx, const int32_t
   // just before a first wire's low end (their enc arithmetic can reach the
   // previous run)
   c->h_net_runs.assign(n_nets, 0);
+  c->h_net_box.assign(4 * (size_t)n_nets, 0);
   for (int32_t i = 0; i < n_nets; ++i) {
     int x0 = c->g.X, x1 = -1, y0 = c->g.Y, y1 = -1;
     for (int64_t p = pin_pfx[i]; p < pin_pfx[i + 1]; ++p) {
@@ -834,6 +887,8 @@ int gr_set_nets(gr_ctx* c, int32_t n_nets, const int64_t* pin_pfx, const int32_t
     }
     const int64_t even = (c->g.Z + 1) / 2, odd = c->g.Z / 2;
     c->h_net_runs[i] = (int32_t)std::min<int64_t>(even * (y1 - y0 + 2) + odd * (x1 - x0 + 2), (int64_t)c->g.Z * c->M);
+    int32_t* box = &c->h_net_box[4 * (size_t)i];
+    box[0] = x0; box[1] = x1; box[2] = y0; box[3] = y1;
   }
   c->n_nets = n_nets;
   free_sched(c->sched);
