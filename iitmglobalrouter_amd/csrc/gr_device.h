@@ -548,6 +548,15 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Net classes handled by one k_tree_small launch: class 0 = 2 pins (one lane
+// per net, 256 nets per block); classes 1..4 = up to 8/16/32/64 pins (a group
+// of G lanes per net, 256/G nets per block).  Block ranges per class.
+struct SmallClasses {
+  const int32_t* list[5];
+  int n[5];
+  int block0[6];  // first block of each class; block0[5] = grid size
+};
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

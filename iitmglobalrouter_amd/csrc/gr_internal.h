@@ -48,6 +48,16 @@ struct BatchMeta {
   int64_t pair_base;      // first pair of the batch in the schedule's pair order
 };
 
+// One batch's fused launch (k_batch): whether it applies, its items in the
+// schedule's item array, and the mixed-tree geometry it shares.
+struct FusedBatch {
+  bool ok = false;
+  int32_t item_ofs = 0, n_items = 0, qlen = 0, n_tree = 0;
+  int32_t cx_ofs = 0;  // first of the batch's 8 per-XCD small-net classes in d_cx
+  int32_t W = 0, cap = 0, scap = 0;
+  size_t lds = 0;
+};
+
 struct Sched {
   std::vector<BatchMeta> meta;
   int32_t n_pos = 0;
@@ -84,8 +94,21 @@ struct Sched {
   size_t cap_ends = 0;
   bool ends_ok = false;
   std::vector<int32_t> h_ofst, h_ids;  // host copy of the schedule (rebuilt when the tuning changes)
+  std::vector<int32_t> h_list, h_deg;  // host copies of the class lists and pin counts per position
+  std::vector<int64_t> h_pair;         // and of each position's first pair in its batch
   hipGraphExec_t exec = nullptr;  // whole schedule as one graph (built on first full replay)
   bool graph_failed = false;
+  // fused batch launches (tuning fork = 4): per-batch geometry and the work
+  // items of every batch, built for the tuning in fused_tune
+  std::vector<FusedBatch> fused;
+  gr_tuning fused_tune{};
+  bool fused_built = false;
+  int4* d_items = nullptr;
+  size_t cap_items = 0;
+  int32_t* d_fcnt = nullptr;  // the fused launch's counters and error word (FusedArgs::cnt)
+  size_t cap_fcnt = 0;
+  grd::SmallClasses* d_cx = nullptr;  // per batch, per XCD: the small-net classes of its fused launch
+  size_t cap_cx = 0;
   // ext: every device array above, and the pair buffers below, point into the
   // allocation-free entry point's static array (never freed, never grown)
   bool ext = false;

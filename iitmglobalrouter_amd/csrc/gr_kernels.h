@@ -26,11 +26,14 @@
 
 namespace grd {
 
-// Phase markers of the workgroup tree (no-ops unless a profiling build
-// defines them).
+// Phase markers of the workgroup tree and of the fused batch launch's items
+// (no-ops unless a profiling build defines them).
 #ifndef GR_TREE_MARK
 #define GR_TREE_MARK(id)
 #define GR_TREE_MARK_INIT
+#endif
+#ifndef GR_ITEM_MARK
+#define GR_ITEM_MARK(phase, kind, xcd)
 #endif
 
 // ---------------------------------------------------------------------------
@@ -89,8 +92,11 @@ __global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restri
                                                  const int32_t* __restrict__ ctr, int all,
                                                  float* __restrict__ remf, int model,
                                                  int16_t* __restrict__ remz, float* __restrict__ vterm,
-                                                 const int32_t* __restrict__ rl, int nrl) {
+                                                 const int32_t* __restrict__ rl, int nrl,
+                                                 int32_t* __restrict__ zero8) {
   extern __shared__ float sh[];
+  // the claim counters of the batch's fused launch (null: none)
+  if (zero8 && blockIdx.x == 0 && threadIdx.x < 8) zero8[threadIdx.x] = 0;
   // lowest marked position of the run, in two slots that alternate between
   // the runs a workgroup processes: a slot is cleared one run before its use
   __shared__ int first_hit[2];
@@ -1270,14 +1276,6 @@ __device__ __forceinline__ void tree_group(const TreeArgs& A, int32_t bs, const 
   }
 }
 
-// Net classes handled by one k_tree_small launch: class 0 = 2 pins (one lane
-// per net, 256 nets per block); classes 1..4 = up to 8/16/32/64 pins (a group
-// of G lanes per net, 256/G nets per block).  Block ranges per class.
-struct SmallClasses {
-  const int32_t* list[5];
-  int n[5];
-  int block0[6];  // first block of each class; block0[5] = grid size
-};
 
 template <int G>
 __device__ __forceinline__ void run_group_class(const TreeArgs& A, int32_t bs, const SmallClasses& C, int cls, int b,
@@ -1417,6 +1415,146 @@ __global__ __launch_bounds__(1024) void k_tree_mixed(TreeArgs A0, SmallClasses C
   } else {
     if ((int)(threadIdx.x >> 6) >= W) return;
     small_block(A, bs, C, (int)blockIdx.x - n_large, W, (GroupLds*)shl);
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// One launch per batch: costing, trees and commit (tuning fork = 4)
+// ---------------------------------------------------------------------------
+
+// Work items of a fused batch launch.  The batch's work is split into 8
+// queues, one per XCD; a workgroup reads the XCC_ID of the XCD it runs on and
+// claims the next item of that XCD's queue (the dispatcher deals the grid's
+// workgroups to the XCDs in turn, so a grid of 8 * qlen gives every XCD qlen
+// workgroups).  Each item depends only on items of its own queue, so a
+// dependency is met through that XCD's L2 without writing the L2 back:
+//   kind 0  costing chunk: pairs [a, a + b) of the batch, then one count on
+//           large-net counter c, or on this XCD's small-net counter (c < 0)
+//   kind 1  large-net tree: batch position a, once its b chunks are counted
+//           on counter c
+//   kind 2  small-net tree block a of this XCD's classes, once the XCD's b
+//           small-net chunks are counted; c = the XCD's small blocks
+//   kind 3  nothing (pads the queues to one length)
+// A tree waits only for chunks claimed before it from its own queue, and
+// chunks wait for nothing, so every wait ends: the chunks it waits for are
+// held by running workgroups.  The wait is bounded all the same (a time-out
+// sets the error word, which the host reports).  The claim counters are
+// cleared by the batch's refresh launch.
+struct FusedArgs {
+  const int4* items;       // queue x at items[x * qlen]
+  int32_t qlen;
+  int32_t* heads;          // per XCD: items claimed so far
+  const SmallClasses* cx;  // the small-net classes of each XCD
+  int32_t* cnt;            // [0] trees finished, [1] error word, [2 + x] XCD x's small chunks finished,
+                           // [10 + x] XCD x's small blocks past their wait, [18 + x] XCD x claim counter (heads), [26 + k] large net k's chunks finished
+  int32_t n_tree;          // tree items of the batch (kinds 1 and 2)
+  const uint2* ends;       // the batch's endpoint table
+  const float* pfx;
+  float* out_cost;         // the batch's pair costs and patterns (the trees read them)
+  int16_t* out_pat;
+  int32_t* ctr;            // the device batch counter
+};
+constexpr int kSpinLimit = 1 << 22;  // s_sleep(2) rounds: about a quarter of a second
+// s_waitcnt operand (gfx9 encoding): vmcnt 0, expcnt and lgkmcnt not waited on
+constexpr int kWaitVm0 = 0x0F70;
+// s_getreg operand: HW_REG_XCC_ID (20), bits [3:0]
+constexpr int kHwRegXccId = 20 | (3 << 11);
+
+// Costing of one chunk: every wavefront waits for its stores to reach the
+// L2 (vmcnt 0: the workgroup barrier does not wait for global stores), then
+// one count (a relaxed agent-scope atomic) publishes them to the XCD.
+template <int ZT, bool BUF>
+__device__ __forceinline__ void fused_chunk(const Geo& g, const FusedArgs& F, int4 it, int xcd) {
+  for (int t = threadIdx.x; t < it.y; t += blockDim.x) {
+    const int64_t w = (int64_t)it.x + t;
+    const uint2 e = F.ends[w];
+    int z1, x1, y1, z2, x2, y2;
+    unpack_cell(e.x, z1, x1, y1);
+    unpack_cell(e.y, z2, x2, y2);
+    const uint64_t best = lane_best_pts<ZT, BUF>(g, F.pfx, z1, x1, y1, z2, x2, y2);
+    F.out_cost[w] = key_cost((uint32_t)(best >> 32));
+    F.out_pat[w] = (int16_t)(uint32_t)best;
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add(it.z >= 0 ? &F.cnt[26 + it.z] : &F.cnt[2 + xcd], 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int ZT, bool BUF>
+__global__ __launch_bounds__(1024) void k_batch(TreeArgs A0, FusedArgs F, int cap, int scap, int W) {
+  const TreeArgs& A = A0;
+  extern __shared__ unsigned long long shl[];
+  __shared__ int wcnt[((kLargeDeg + 1023) / 1024) * 16];
+  __shared__ int s_sh[2];
+  __shared__ int32_t s_bs;
+  __shared__ int s_claim[2];
+  if (threadIdx.x == 0) {
+    const int x = (int)(__builtin_amdgcn_s_getreg(kHwRegXccId) & 7);
+    const int j = __hip_atomic_fetch_add(&F.heads[x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_claim[0] = x;
+    s_claim[1] = j < F.qlen ? x * F.qlen + j : -1;
+  }
+  __syncthreads();
+  const int xcd = s_claim[0];
+  if (s_claim[1] < 0) return;  // more workgroups than items on this XCD
+  const int4 item = F.items[s_claim[1]];
+  const int kind = item.x;
+  const int4 it = make_int4(item.y, item.z, item.w, 0);
+  if (kind == 3) return;
+  GR_ITEM_MARK(0, kind, xcd);
+  if (kind == 0) {
+    fused_chunk<ZT, BUF>(A.g, F, it, xcd);
+    GR_ITEM_MARK(2, kind, xcd);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    // relaxed polls, then an L1 invalidation: this CU's L1 may hold lines of
+    // the pair arrays read before the chunks wrote them
+    int32_t* cp = kind == 1 ? &F.cnt[26 + it.z] : &F.cnt[2 + xcd];
+    int spins = 0;
+    while (__hip_atomic_load(cp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < it.y) {
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(&F.cnt[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    // the chunks' stores are in this XCD's L2; invalidate only this CU's L1
+    // (an agent-scope acquire would also invalidate the L2 the XCD's other
+    // workgroups are costing from).  The memory clobber keeps the pair loads
+    // below it.
+    asm volatile("buffer_inv sc0" ::: "memory");
+    // every chunk this item waited for is in: its counter can start over
+    // for the next batch (small-net: once every small block of the XCD is past)
+    if (kind == 1) {
+      __hip_atomic_store(cp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (__hip_atomic_fetch_add(&F.cnt[10 + xcd], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+               it.z - 1) {
+      __hip_atomic_store(cp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&F.cnt[10 + xcd], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_bs = __hip_atomic_load(F.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;  // published below
+  }
+  __syncthreads();
+  GR_ITEM_MARK(1, kind, xcd);
+  const int32_t bs = s_bs;
+  if (kind == 1) {
+    const TreeLds L = tree_lds(shl, cap, 0, scap, wcnt, s_sh);
+    tree_commit<false>(A, bs, it.x, threadIdx.x, blockDim.x, L);
+  } else if ((int)(threadIdx.x >> 6) < W) {
+    small_block(A, bs, F.cx[xcd], it.x, W, (GroupLds*)shl);
+  }
+  // the batch's last tree publishes its stamp for the refresh (every tree
+  // read the old counter before counting itself)
+  __syncthreads();
+  GR_ITEM_MARK(2, kind, xcd);
+  if (threadIdx.x == 0 &&
+      __hip_atomic_fetch_add(&F.cnt[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == F.n_tree - 1) {
+    __hip_atomic_store(F.ctr, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&F.cnt[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -121,7 +121,8 @@ int refresh_grid(const gr_ctx* c, int runs) {
 
 // rl (nrl entries): the batch's candidate runs, one workgroup each up to the
 // tuned grid; null: every run, with the grid bounded by runs.
-void launch_refresh(gr_ctx* c, int all, int runs = -1, const int32_t* rl = nullptr, int nrl = 0) {
+void launch_refresh(gr_ctx* c, int all, int runs = -1, const int32_t* rl = nullptr, int nrl = 0,
+                    int32_t* zero8 = nullptr) {
   const int grid = rl ? std::max(1, std::min(nrl, c->tune.refresh_blocks)) : refresh_grid(c, runs);
   if (c->prefix)
     hipLaunchKernelGGL(k_refresh_prefix, dim3(grid), dim3(256), refresh_prefix_lds(c), c->stream,
@@ -130,13 +131,13 @@ void launch_refresh(gr_ctx* c, int all, int runs = -1, const int32_t* rl = nullp
   else if (c->d_remf)
     hipLaunchKernelGGL(k_refresh<true>, dim3(grid), dim3(256), refresh_lds(c, false), c->stream,
                        c->g, c->M, c->d_rem, c->d_mark, c->d_pfx, c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf,
-                       c->model, (int16_t*)nullptr, (float*)nullptr, rl, nrl);
+                       c->model, (int16_t*)nullptr, (float*)nullptr, rl, nrl, zero8);
   else
     hipLaunchKernelGGL((c->vterm_live ? k_refresh<false, true> : k_refresh<false, false>), dim3(grid),
                        dim3(256), refresh_lds(c, c->vterm_live), c->stream, c->g, c->M, c->d_rem, c->d_mark, c->d_pfx,
                        c->d_lut, c->d_run_stamp, c->d_ctr, all, c->d_remf, c->model,
                        c->remz_live ? c->d_remz : (int16_t*)nullptr, c->vterm_live ? c->d_vterm : (float*)nullptr,
-                       rl, nrl);
+                       rl, nrl, zero8);
 }
 
 int refresh_all(gr_ctx* c) {
@@ -349,6 +350,10 @@ int build_sched(gr_ctx* c, Sched& s, int32_t n_batches, const int32_t* batch_ofs
   HostSched h;
   int rc = host_sched(c, s, h, n_batches, batch_ofst, net_ids, true);
   if (rc) return rc;
+  s.h_list = h.list;
+  s.h_deg = h.degv;
+  s.h_pair = h.pair;
+  s.fused_built = false;
   if (h.rlist.size() > s.cap_rlist) {
     if ((rc = dalloc(c, &s.d_rlist, h.rlist.size()))) return rc;
     s.cap_rlist = h.rlist.size();
@@ -466,6 +471,22 @@ EndsFn ends_kernel_for(int Z) {
     default: return nullptr;
   }
 }
+typedef void (*BatchFn)(TreeArgs, FusedArgs, int, int, int);
+BatchFn batch_kernel_for(int Z) {
+  switch (Z) {
+    case 3: return k_batch<3, true>;
+    case 4: return k_batch<4, true>;
+    case 5: return k_batch<5, true>;
+    case 6: return k_batch<6, true>;
+    case 7: return k_batch<7, true>;
+    case 8: return k_batch<8, true>;
+    case 9: return k_batch<9, true>;
+    case 10: return k_batch<10, true>;
+    case 11: return k_batch<11, true>;
+    case 12: return k_batch<12, true>;
+    default: return nullptr;
+  }
+}
 // Buffer-resource loads when the prefix (one padding float and a float per
 // cell) spans fewer than 2^31 bytes, flat loads otherwise.
 bool prefix_buf(const Geo& g) { return ((uint64_t)g.XY * (uint64_t)g.Z + 1) * 4 <= 0x7fffffffull; }
@@ -532,7 +553,7 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
     int W = std::min(8, Wmax);
     size_t lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
     bool fits = 2 * (lds + 1024) <= (size_t)c->lds_max;
-    if (!fits && c->tune.fork == 3) {
+    if (!fits && c->tune.fork >= 3) {
       W = (int)std::min<size_t>((size_t)Wmax, std::max<size_t>(8, large_lds / sizeof(GroupLds)));
       lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
       fits = lds + 1024 <= (size_t)c->lds_max;
@@ -569,11 +590,277 @@ int launch_trees(gr_ctx* c, const Sched& s, const BatchMeta& m, TreeArgs A) {
   return 0;
 }
 
+// Work items of every batch's fused launch (k_batch, tuning fork = 4) for
+// the current tuning, uploaded once; a batch the fused launch does not cover
+// (square key matrices, nets above kLargeDeg pins, no LDS fit) keeps the
+// separate costing and tree launches.  Each batch's work is split into one
+// queue per XCD, balanced by estimated workgroup time (a chunk of costing
+// about 10 us, a large tree 5 + 0.12 n us for n pins, a small-net block about
+// 13 us):
+//   small nets: 8 runs of consecutive positions of about equal estimated
+//   time; XCD x costs its run's pairs in chunks (one pair per thread of the
+//   workgroup) that do not cross a large net's pairs, and runs the tree blocks
+//   of the run's nets (the class lists restricted to the run);
+//   large nets: by descending pin count, each to the XCD with the least time
+//   so far; the XCD costs the net's pairs in chunks and runs its tree.
+// A queue is [chunks of its largest net][that net's tree][the small nets'
+// chunks][the other large nets' chunks][their trees][the small tree blocks]:
+// the largest tree, the batch's critical path, starts once its own chunks are
+// done, and every other tree is claimed after all chunks of the queue, so it
+// seldom waits holding a CU.
+int build_fused(gr_ctx* c, Sched& s) {
+  s.fused.assign(s.meta.size(), FusedBatch{});
+  s.fused_tune = c->tune;
+  s.fused_built = true;
+  if (c->tune.fork != 4 || &s != &c->sched || !s.ends_ok || c->prefix || c->tune.lpat_wave || !prefix_buf(c->g) ||
+      !batch_kernel_for(c->g.Z))
+    return 0;
+  const int T = c->tune.large_threads;
+  const double kChunkUs = 10.0, kBlockUs = 13.0;
+  std::vector<int4> items;
+  std::vector<SmallClasses> cxs;
+  int max_large = 0;
+  for (size_t b = 0; b < s.meta.size(); ++b) {
+    const BatchMeta& m = s.meta[b];
+    FusedBatch& f = s.fused[b];
+    if (m.n == 0 || m.P <= 0 || m.P > INT32_MAX || m.streams || m.cls_n[kClsGiant]) continue;
+    const int n_large = m.cls_n[kClsLarge];
+    const int scap = n_large ? (int)m.max_large_P : 0;
+    const size_t large_lds = n_large ? tree_lds_bytes(m.max_large, 0, scap) : 0;
+    const int Wmax = T / 64;
+    int W = std::min(8, Wmax);
+    size_t lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+    if (2 * (lds + 1024) > (size_t)c->lds_max) {
+      W = (int)std::min<size_t>((size_t)Wmax, std::max<size_t>(8, large_lds / sizeof(GroupLds)));
+      lds = std::max(large_lds, (size_t)W * sizeof(GroupLds));
+      if (lds + 1024 > (size_t)c->lds_max) continue;
+    }
+    auto npairs = [&](int i) {
+      const int64_t d = s.h_deg[m.begin + i];
+      return d * (d - 1) / 2;
+    };
+    std::vector<uint8_t> large(m.n, 0);
+    for (int q = 0; q < n_large; ++q) large[s.h_list[m.cls_ofs[kClsLarge] + q]] = 1;
+    // estimated workgroup time of a small net: its share of a costing chunk
+    // and of its class's tree block
+    auto small_us = [&](int i) {
+      const int64_t d = s.h_deg[m.begin + i];
+      const int k = net_class(d, c->tune.group_max);
+      const int per_block = k == 0 ? 64 * W : 64 * W / kClsMax[k];
+      return (double)npairs(i) * kChunkUs / T + kBlockUs / per_block;
+    };
+    // small nets: 8 position runs [cut[x], cut[x+1]) of about equal time
+    double small_total = 0;
+    for (int i = 0; i < m.n; ++i)
+      if (!large[i]) small_total += small_us(i);
+    int cut[9];
+    cut[0] = 0;
+    {
+      double acc = 0;
+      int x = 1;
+      for (int i = 0; i < m.n && x < 8; ++i) {
+        if (!large[i]) acc += small_us(i);
+        while (x < 8 && acc * 8 >= small_total * x) cut[x++] = i + 1;
+      }
+      while (x < 8) cut[x++] = m.n;
+    }
+    cut[8] = m.n;
+    double load[8];
+    for (int x = 0; x < 8; ++x) {
+      load[x] = 0;
+      for (int i = cut[x]; i < cut[x + 1]; ++i)
+        if (!large[i]) load[x] += small_us(i);
+    }
+    auto chunks = [&](std::vector<int4>& dst, int64_t p0, int64_t np, int dep) {
+      int k = 0;
+      for (int64_t o = 0; o < np; o += T, ++k)
+        dst.push_back(make_int4(0, (int)(p0 + o), (int)std::min<int64_t>(T, np - o), dep));
+      return k;
+    };
+    // large nets to XCDs, largest first
+    std::vector<int32_t> order(s.h_list.begin() + m.cls_ofs[kClsLarge],
+                               s.h_list.begin() + m.cls_ofs[kClsLarge] + n_large);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b2) { return s.h_deg[m.begin + a] > s.h_deg[m.begin + b2]; });
+    std::vector<int> lx[8];  // large-net indices (into order) of each XCD, largest first
+    for (int k = 0; k < n_large; ++k) {
+      const int32_t i = order[k];
+      int x = 0;
+      for (int y = 1; y < 8; ++y)
+        if (load[y] < load[x]) x = y;
+      const int64_t np = npairs(i);
+      load[x] += (double)((np + T - 1) / T) * kChunkUs + 5.0 + 0.12 * (double)s.h_deg[m.begin + i];
+      lx[x].push_back(k);
+    }
+    std::vector<int4> q[8];
+    int n_tree = n_large;
+    const size_t cx0 = cxs.size();
+    for (int x = 0; x < 8; ++x) {
+      std::vector<int> nch(lx[x].size(), 0);
+      // the largest net's chunks and tree
+      if (!lx[x].empty()) {
+        const int k = lx[x][0];
+        nch[0] = chunks(q[x], s.h_pair[m.begin + order[k]], npairs(order[k]), k);
+        q[x].push_back(make_int4(1, order[k], nch[0], k));
+      }
+      // the small nets' chunks over the run's pairs between large nets
+      int nsc = 0;
+      int64_t run0 = -1, run_n = 0;
+      for (int i = cut[x]; i <= cut[x + 1]; ++i) {
+        if (i == cut[x + 1] || large[i]) {
+          if (run_n) nsc += chunks(q[x], run0, run_n, -1);
+          run0 = -1;
+          run_n = 0;
+          continue;
+        }
+        if (run0 < 0) run0 = s.h_pair[m.begin + i];
+        run_n += npairs(i);
+      }
+      // the other large nets' chunks, then their trees
+      for (size_t j = 1; j < lx[x].size(); ++j) {
+        const int k = lx[x][j];
+        nch[j] = chunks(q[x], s.h_pair[m.begin + order[k]], npairs(order[k]), k);
+      }
+      for (size_t j = 1; j < lx[x].size(); ++j) {
+        const int k = lx[x][j];
+        q[x].push_back(make_int4(1, order[k], nch[j], k));
+      }
+      // the small tree blocks over the class lists restricted to the run
+      SmallClasses Cx;
+      int mb = 0;
+      for (int k = 0; k < 5; ++k) {
+        const int32_t* lb = s.h_list.data() + m.cls_ofs[k];
+        const int32_t* le = lb + m.cls_n[k];
+        const int32_t* a = std::lower_bound(lb, le, cut[x]);
+        const int32_t* e = std::lower_bound(lb, le, cut[x + 1]);
+        Cx.list[k] = s.d_list + m.cls_ofs[k] + (a - lb);
+        Cx.n[k] = (int)(e - a);
+        Cx.block0[k] = mb;
+        const int per_block = k == 0 ? 64 * W : 64 * W / kClsMax[k];
+        mb += (Cx.n[k] + per_block - 1) / per_block;
+      }
+      Cx.block0[5] = mb;
+      cxs.push_back(Cx);
+      for (int bq = 0; bq < mb; ++bq) q[x].push_back(make_int4(2, bq, nsc, mb));
+      n_tree += mb;
+      if (mb == 0 && nsc) return fail(c, GR_ESTATE, "fused batch: small-net chunks without tree blocks");
+    }
+    // queue x at [at + x * len, at + (x + 1) * len), padded to one length
+    size_t len = 0;
+    for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
+    const size_t at = items.size();
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = 0; j < len; ++j) items.push_back(j < q[x].size() ? q[x][j] : make_int4(3, 0, 0, 0));
+    // the chunks tile the batch's pairs exactly once
+    std::vector<std::pair<int32_t, int32_t>> tiles;
+    for (size_t j = at; j < items.size(); ++j)
+      if (items[j].x == 0) tiles.emplace_back(items[j].y, items[j].z);
+    std::sort(tiles.begin(), tiles.end());
+    int64_t next = 0;
+    for (const auto& t : tiles) {
+      if (t.first != next || t.second <= 0) return fail(c, GR_ESTATE, "fused batch items do not tile the pairs");
+      next += t.second;
+    }
+    if (next != m.P) return fail(c, GR_ESTATE, "fused batch items do not tile the pairs");
+    f.item_ofs = (int32_t)at;
+    f.n_items = (int32_t)(items.size() - at);
+    f.qlen = (int32_t)len;
+    f.n_tree = n_tree;
+    f.cx_ofs = (int32_t)cx0;
+    f.W = W;
+    f.cap = m.max_large;
+    f.scap = scap;
+    f.lds = lds;
+    f.ok = n_tree > 0;
+    max_large = std::max(max_large, n_large);
+  }
+  int rc;
+  if (items.size() > s.cap_items) {
+    if ((rc = dalloc(c, &s.d_items, items.size()))) return rc;
+    s.cap_items = items.size();
+  }
+  if (cxs.size() > s.cap_cx) {
+    if ((rc = dalloc(c, &s.d_cx, cxs.size()))) return rc;
+    s.cap_cx = cxs.size();
+  }
+  if ((size_t)max_large + 26 > s.cap_fcnt) {
+    if ((rc = dalloc(c, &s.d_fcnt, (size_t)max_large + 26))) return rc;
+    s.cap_fcnt = (size_t)max_large + 26;
+  }
+  if (!items.empty())
+    HIPC(c, hipMemcpy(s.d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  if (!cxs.empty())
+    HIPC(c, hipMemcpy(s.d_cx, cxs.data(), sizeof(SmallClasses) * cxs.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(s.d_fcnt, 0, sizeof(int32_t) * s.cap_fcnt));
+  return 0;
+}
+
+bool fused_applies(const gr_ctx* c, const Sched& s, int b) {
+  return c->tune.fork == 4 && s.fused_built && !c->prop && !c->collect && (size_t)b < s.fused.size() && s.fused[b].ok;
+}
+
 int enqueue_batch(gr_ctx* c, Sched& s, int b) {
   const BatchMeta& m = s.meta[b];
   if (m.n == 0) return 0;
   float* pcost = s.ext ? s.x_pair_cost : c->d_pair_cost;
   int16_t* ppat = s.ext ? s.x_pair_pat : c->d_pair_pat;
+  if (fused_applies(c, s, b)) {
+    const FusedBatch& f = s.fused[b];
+    TreeArgs A;
+    A.g = c->g;
+    A.list = nullptr;
+    A.n_list = 0;
+    A.pinofs = s.d_pinofs + m.begin;
+    A.deg = s.d_deg + m.begin;
+    A.slot = s.d_slot + m.begin;
+    A.pair = s.d_pair + m.begin;
+    A.cost_base = m.begin;
+    A.pins = c->d_pins;
+    A.pair_cost = pcost;
+    A.pair_pat = ppat;
+    A.out_edges = s.d_edges;
+    A.out_lpat = s.d_lpat;
+    A.out_cost = s.d_cost;
+    A.mark = c->d_mark;
+    A.run_stamp = c->d_run_stamp;
+    A.ctr = c->d_ctr;
+    A.quirk = c->quirk;
+    A.M = c->M;
+    A.true_mst = c->true_mst;
+    A.prefix = 0;
+    A.vhalf = nullptr;
+    A.sqofs = nullptr;
+    A.sq = nullptr;
+    A.rowmin = nullptr;
+    A.rowmin_base = 0;
+    FusedArgs F;
+    F.items = s.d_items + f.item_ofs;
+    F.qlen = f.qlen;
+    F.heads = s.d_fcnt + 18;
+    F.cx = s.d_cx + f.cx_ofs;
+    F.cnt = s.d_fcnt;
+    F.n_tree = f.n_tree;
+    F.ends = s.d_ends + m.pair_base;
+    F.pfx = c->d_pfx;
+    F.out_cost = pcost;
+    F.out_pat = ppat;
+    F.ctr = c->d_ctr;
+    {
+      KTimer t(c, GR_K_TREE);
+      hipLaunchKernelGGL(batch_kernel_for(c->g.Z), dim3(f.n_items), dim3(c->tune.large_threads), f.lds, c->stream, A,
+                         F, f.cap, f.scap, f.W);
+    }
+    {
+      KTimer t(c, GR_K_REFRESH);
+      if (m.rlist_ofs >= 0 && s.d_rlist) launch_refresh(c, 0, m.runs, s.d_rlist + m.rlist_ofs, m.nrl, F.heads);
+      else launch_refresh(c, 0, m.runs, nullptr, 0, F.heads);
+    }
+    HIPC(c, hipGetLastError());
+    c->last_P = m.P;
+    c->last_pair_cost = pcost;
+    c->last_pair_pat = ppat;
+    return 0;
+  }
   // square key matrices: written by the exhaustive BIT costing kernels only
   SquareOut sq{nullptr, nullptr, nullptr};
   if (m.streams && s.d_sq && !c->prop && !c->prefix) sq = SquareOut{s.d_sqofs + m.begin, s.d_deg + m.begin, s.d_sq};
@@ -675,6 +962,7 @@ void free_sched(Sched& s) {
   (void)hipFree(s.d_chunk); (void)hipFree(s.d_edges); (void)hipFree(s.d_lpat); (void)hipFree(s.d_cost);
   (void)hipFree(s.d_giant); (void)hipFree(s.d_sqofs); (void)hipFree(s.d_sq); (void)hipFree(s.d_rowtask);
   (void)hipFree(s.d_rowmin); (void)hipFree(s.d_ends); (void)hipFree(s.d_rlist);
+  (void)hipFree(s.d_items); (void)hipFree(s.d_fcnt); (void)hipFree(s.d_cx);
   s = Sched();
 }
 
@@ -794,6 +1082,9 @@ int create_ctx(const gr_grid_desc* d, const int32_t* cap, const float* capf, gr_
                           (int)c->lds_max - 2048) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_tree_giant, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)c->lds_max - 4096) != hipSuccess ||
+      (batch_kernel_for(c->g.Z) &&
+       hipFuncSetAttribute((const void*)batch_kernel_for(c->g.Z), hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)c->lds_max - 2048) != hipSuccess) ||
       hipMemset(c->d_ctr, 0, sizeof(int32_t)) != hipSuccess ||
       hipMemset(c->d_mark, 0, sizeof(uint8_t) * cs) != hipSuccess ||
       hipMemset(c->d_run_stamp, 0, sizeof(int32_t) * nr) != hipSuccess ||
@@ -998,6 +1289,10 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
     return fail(c, GR_EINVAL, "batch range outside the schedule");
   (void)hipSetDevice(c->dev);
   Sched& s = c->sched;
+  if (!s.fused_built) {
+    const int rc = build_fused(c, s);
+    if (rc) return rc;
+  }
   const bool whole = first == 0 && count == (int32_t)s.meta.size() && count > 0;
   if (whole && !c->timing && (!c->prop || mlp_fused(c->prop)) && !c->collect && !s.graph_failed) {
     if (!s.exec) {
@@ -1036,6 +1331,14 @@ int gr_route_schedule(gr_ctx* c, int32_t first, int32_t count) {
 int gr_sync(gr_ctx* c) {
   if (!c) return fail(c, GR_EINVAL, "null context");
   HIPC(c, hipStreamSynchronize(c->stream));
+  if (c->sched.d_fcnt) {
+    int32_t err = 0;
+    HIPC(c, hipMemcpy(&err, c->sched.d_fcnt + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (err) {
+      (void)hipMemset(c->sched.d_fcnt, 0, sizeof(int32_t) * c->sched.cap_fcnt);
+      return fail(c, GR_ESTATE, "a fused batch launch timed out waiting for its costing chunks; its routes are invalid");
+    }
+  }
   return 0;
 }
 
@@ -1141,7 +1444,7 @@ int gr_get_tuning(const gr_ctx* c, gr_tuning* t) {
 int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
   if (!c || !t) return fail(c, GR_EINVAL, "null argument");
   if ((t->large_threads != 256 && t->large_threads != 512 && t->large_threads != 1024) ||
-      (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || t->fork < 0 || t->fork > 3 ||
+      (t->group_max != 32 && t->group_max != 64) || t->refresh_blocks < 1 || t->fork < 0 || t->fork > 4 ||
       (t->lpat_wave != 0 && t->lpat_wave != 1) ||
       (t->lpat_block != 0 && t->lpat_block != 64 && t->lpat_block != 128 && t->lpat_block != 256))
     return fail(c, GR_EINVAL, "tuning out of range");
@@ -1153,6 +1456,7 @@ int gr_set_tuning(gr_ctx* c, const gr_tuning* t) {
   Sched& s = c->sched;
   if (s.exec) { (void)hipGraphExecDestroy(s.exec); s.exec = nullptr; }
   s.graph_failed = false;
+  s.fused_built = false;
   if (reclass && !s.meta.empty()) {
     const std::vector<int32_t> ofst = s.h_ofst, ids = s.h_ids;
     return build_sched(c, s, (int32_t)ofst.size() - 1, ofst.data(), ids.data());
@@ -1206,7 +1510,7 @@ int gr_autotune(gr_ctx* c, int32_t reps, gr_tuning* best, double* best_ms) {
   };
   for (int gm : {64, 32})
     for (int lt : {1024, 512, 256})
-      for (int fk : {3, 1, 0})
+      for (int fk : {4, 3, 1, 0})
         for (int rb : {1024, 2048, 4096, 8192})
           for (int lw : {0, 1}) {
             if (lw && !lane_kernel(c->g)) continue;  // only the wave kernel exists for this Z

@@ -243,7 +243,12 @@ int gr_lut_size(void);
  *                   0: two launches in sequence; 1: the large-net launch on a second
  *                   stream, parallel to the small-net one; 2: one launch for both when
  *                   two of its workgroups fit a CU's LDS, else as 1; 3: one launch
- *                   for both whenever the large tree fits in LDS
+ *                   for both whenever the large tree fits in LDS; 4: one launch per
+ *                   batch for costing, trees and commit (k_batch: per-XCD queues of
+ *                   costing chunks and trees, each tree waiting on its own XCD's
+ *                   chunks), on the resident schedule in the exhaustive BIT modes,
+ *                   else as 3.  gr_sync reports a fused launch whose bounded wait
+ *                   timed out (GR_ESTATE)
  *   refresh_blocks  persistent grid of the cost refresh (>= 1)
  *   lpat_wave       1: one wavefront per pin pair instead of one lane per pair
  *   lpat_block      workgroup size of the lane-per-pair costing (64, 128 or 256), or
