@@ -591,6 +591,55 @@ __device__ __forceinline__ void commit_edge(const TreeArgs& A, int32_t bs, uint3
   if (z != z2) mark_via(A, bs, z, z2, x, y);
 }
 
+// One of the five parts of commit_edge's cell sequence for a BIT-mode edge
+// (the same cells, so a net's edges can be marked part by part on separate
+// threads; marks and stamps are idempotent stores):
+//   0 the via stack z1 -> a at the source, with the quirk cells before the
+//     first wire's low end;  1 the first wire on layer a;  2 the via stack
+//   a -> b' at the corner;  3 the second wire on b';  4 the via stack to z2.
+__device__ __forceinline__ void commit_part(const TreeArgs& A, int32_t bs, int part, int z1, int x1, int y1, int z2,
+                                            int x2, int y2, int p) {
+  const Geo& g = A.g;
+  if (p < g.Z2 || p >= g.Z * g.Z2) return;
+  const int a = p / g.Z2, b = p - a * g.Z2;
+  const bool D = (x1 != x2) && (y1 != y2);
+  if (part == 0) {
+    if (z1 == a) return;
+    mark_via(A, bs, z1, a, x1, y1);
+    if (A.quirk) {
+      const int k = (z1 < a ? a - z1 : z1 - a) - 1;
+      const uint32_t e0 = (a & 1) ? enc(g, a, x1, min(y1, y2)) : enc(g, a, min(x1, x2), y1);
+      for (int t = 1; t <= k && (uint32_t)t <= e0; ++t) mark_any(A, bs, e0 - t);
+    }
+    return;
+  }
+  if (part == 1) {
+    if (a & 1) mark_wire(A, bs, a, x1, min(y1, y2), max(y1, y2));
+    else       mark_wire(A, bs, a, y1, min(x1, x2), max(x1, x2));
+    return;
+  }
+  // after the first wire
+  const int x = (a & 1) ? x1 : x2, y = (a & 1) ? y2 : y1;
+  const int bp = (a & 1) ? 2 * b : 2 * b + 1;
+  const bool two = D && bp != 0;
+  if (part == 2) {
+    if (two) mark_via(A, bs, a, bp, x, y);
+    return;
+  }
+  if (part == 3) {
+    if (!two) return;
+    if (bp & 1) mark_wire(A, bs, bp, x, min(y, y2), max(y, y2));
+    else        mark_wire(A, bs, bp, y, min(x, x2), max(x, x2));
+    return;
+  }
+  int z = a, xf = x, yf = y;
+  if (two) {
+    z = bp;
+    if (bp & 1) yf = y2; else xf = x2;
+  }
+  if (z != z2) mark_via(A, bs, z, z2, xf, yf);
+}
+
 // Net cost: the n-1 accepted minimum costs added in slot order (the
 // reference's serial net_dist += cost), the LDS loads issued in groups of 8
 // ahead of the dependent adds.
@@ -1141,7 +1190,36 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
   }
   const int64_t sb = A.slot[pos];
   const int32_t* npins = A.pins + pb;
-  for (int e = tid; e < n - 1; e += gsize) emit_edge(A, bs, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  if (A.prefix) {
+    for (int e = tid; e < n - 1; e += gsize) emit_edge(A, bs, pos, sb, e, edges[e].x, edges[e].y, pat, npins);
+  } else {
+    // outputs and each edge's endpoints and pattern (in best and partner,
+    // free after the rounds), then the commit marks as (part, edge) tasks
+    // over every thread: a wire no longer waits for the vias and the other
+    // wire of its edge on the same thread
+    for (int e = tid; e < n - 1; e += gsize) {
+      const int on1 = edges[e].x, on2 = edges[e].y;
+      A.out_edges[sb + e] = ((int64_t)pos << 30) | ((int64_t)on1 << 15) | (int64_t)on2;
+      const int hi = on1 > on2 ? on1 : on2, lo = on1 > on2 ? on2 : on1;
+      const int p = pat[(uint32_t)((int64_t)hi * (hi - 1) / 2 + lo)];
+      A.out_lpat[sb + e] = p;
+      int z1, x1, y1, z2, x2, y2;
+      dec(A.g, (uint32_t)npins[hi], z1, x1, y1);
+      dec(A.g, (uint32_t)npins[lo], z2, x2, y2);
+      best[e] = (unsigned long long)(uint32_t)x1 | ((unsigned long long)(uint32_t)y1 << 16) |
+                ((unsigned long long)(uint32_t)x2 << 32) | ((unsigned long long)(uint32_t)y2 << 48);
+      partner[e] = z1 | (z2 << 8) | (p << 16);
+    }
+    tree_sync<GN>();
+    const int ne = n - 1;
+    for (int t = tid; t < 5 * ne; t += gsize) {
+      const int part = t / ne, e = t - part * ne;
+      const unsigned long long xy = best[e];
+      const int zz = partner[e];
+      commit_part(A, bs, part, zz & 0xff, (int)(xy & 0xffff), (int)((xy >> 16) & 0xffff), (zz >> 8) & 0xff,
+                  (int)((xy >> 32) & 0xffff), (int)(xy >> 48), zz >> 16);
+    }
+  }
   // the net cost on a thread without an edge when there is one
   if (tid == (n - 1) % gsize) {
     A.out_edges[sb + n - 1] = -1;
