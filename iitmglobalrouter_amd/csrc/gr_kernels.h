@@ -85,7 +85,7 @@ namespace grd {
 __host__ __device__ inline int refresh_stride(int M) { return (M + 3) & ~3; }
 
 template <bool FLT, bool VT = false>
-__global__ __launch_bounds__(256) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FLT ? 6 : 8))) void k_refresh(Geo g, int M, int32_t* __restrict__ rem,
                                                  uint8_t* __restrict__ mark, float* __restrict__ pfx,
                                                  const float* __restrict__ lut,
                                                  const int32_t* __restrict__ run_stamp,
