@@ -258,6 +258,11 @@ int host_sched(gr_ctx* c, Sched& s, HostSched& h, int32_t n_batches, const int32
         if (k == kClsGiant) m.max_giant = std::max<int>(m.max_giant, (int)d);
       }
       m.cls_n[cls] = lo - m.cls_ofs[cls];
+      // workgroup trees by descending pin count: the batch's longest trees
+      // are dispatched first
+      if (cls == kClsLarge || cls == kClsGiant)
+        std::stable_sort(h.list.begin() + m.cls_ofs[cls], h.list.begin() + lo,
+                         [&](int32_t a, int32_t b2) { return h.degv[m.begin + a] > h.degv[m.begin + b2]; });
     }
     // owner of pair 64*c for every 64-pair chunk of the batch
     m.chunk_ofs = (int32_t)h.chunk.size();
