@@ -1393,10 +1393,17 @@ __global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C)
 // of the same round, and partner is written again only in the next round.
 // The component root lists hold the most components a matrix of mcap
 // entries can describe.
+// Components a matrix of mcap entries can describe: the smallest c with
+// c(c-1)/2 > mcap, at most cap + 1.  Closed form with an exact correction
+// (every wavefront of a tree computes it at entry; a counting loop there
+// ran about 200 scalar iterations per wavefront, all of a CU's wavefronts
+// sharing its scalar unit, and delayed the first barrier by up to 16 us).
 __host__ __device__ inline int tree_ccap(int cap, int mcap) {
-  int c = 1;
-  while ((int64_t)c * (c - 1) / 2 <= mcap && c <= cap) ++c;
-  return c;
+  int64_t c = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)mcap)) * 0.5);
+  if (c < 1) c = 1;
+  while (c > 1 && (c - 1) * (c - 2) / 2 > mcap) --c;
+  while (c * (c - 1) / 2 <= mcap) ++c;
+  return (int)(c < (int64_t)cap + 1 ? c : (int64_t)cap + 1);
 }
 __host__ __device__ inline size_t tree_lds_bytes(int cap, int mcap, int scap) {
   return (size_t)cap * 16 + (size_t)mcap * 8 + (size_t)cap * 20 + (size_t)tree_ccap(cap, mcap) * 8 +
