@@ -1149,21 +1149,52 @@ __device__ __forceinline__ void tree_commit(const TreeArgs& A, int32_t bs, int p
       C = tree_ids<GN>(L, n, tid, gsize, croot_cur);
       mcur = L.mat;
       for (uint32_t e = tid; e < Tn; e += gsize) mcur[e] = ~0ull;
+      // every node's component index, read once per pair by the build
+      // (par_a is free until the next round's partner phase rewrites it)
+      int* comp = L.par_a;
+      if (sqk)
+        for (int k = tid; k < n; k += gsize) comp[k] = L.cid[root[k]];
       tree_sync<GN>();
       const uint32_t chunk = (P + gsize - 1) / gsize;
       uint32_t o = (uint32_t)tid * chunk;
       const uint32_t oe = o + chunk < P ? o + chunk : P;
       if (sqk) {
-        // the lower half of the square, row by row (coalesced)
+        // the lower half of the square, row by row: 16-byte loads (4 keys
+        // per lane, 256 columns per wavefront step, two steps in flight);
+        // a lane's consecutive keys into the same component are combined
+        // before the atomic (the minimum over a union is the minimum of the
+        // minima, and within a run the pair index grows with k)
         for (int j = 1 + wv; j < n; j += nwv) {
-          const int cj = L.cid[root[j]];
-          const uint32_t* row = sqk + (size_t)j * (size_t)n4;
+          const int cj = comp[j];
+          const uint4* row = reinterpret_cast<const uint4*>(sqk + (size_t)j * (size_t)n4);
           const uint32_t ob = tri_index(j, 0);
-          for (int k = lane; k < j; k += 64) {
-            const int ck = L.cid[root[k]];
-            if (ck != cj)
-              atomicMin(&mcur[cj > ck ? tri_index(cj, ck) : tri_index(ck, cj)],
-                        ((unsigned long long)row[k] << 32) | (ob + (uint32_t)k));
+          for (int k0 = 0; k0 < j; k0 += 512) {
+            uint4 v[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int k4 = k0 + 256 * u + 4 * lane;
+              v[u] = k4 < j ? row[k4 >> 2] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int k4 = k0 + 256 * u + 4 * lane;
+              const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+              int pc = -1;
+              unsigned long long pm = ~0ull;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int k = k4 + q;
+                const int ck = k < j ? comp[k] : cj;
+                if (ck != pc) {
+                  if (pc >= 0 && pc != cj) atomicMin(&mcur[cj > pc ? tri_index(cj, pc) : tri_index(pc, cj)], pm);
+                  pc = ck;
+                  pm = ~0ull;
+                }
+                const unsigned long long key = ((unsigned long long)kv[q] << 32) | (ob + (uint32_t)k);
+                pm = key < pm ? key : pm;
+              }
+              if (pc >= 0 && pc != cj) atomicMin(&mcur[cj > pc ? tri_index(cj, pc) : tri_index(pc, cj)], pm);
+            }
           }
         }
       } else if (o < oe) {
