@@ -1430,6 +1430,7 @@ __global__ __launch_bounds__(256) void k_tree_small(TreeArgs A0, SmallClasses C)
 // ran about 200 scalar iterations per wavefront, all of a CU's wavefronts
 // sharing its scalar unit, and delayed the first barrier by up to 16 us).
 __host__ __device__ inline int tree_ccap(int cap, int mcap) {
+  if (mcap <= 0) return cap + 1 < 2 ? cap + 1 : 2;  // no matrix: c(c-1)/2 > 0 first at c = 2
   int64_t c = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)mcap)) * 0.5);
   if (c < 1) c = 1;
   while (c > 1 && (c - 1) * (c - 2) / 2 > mcap) --c;
