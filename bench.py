@@ -54,8 +54,9 @@ def footprint(X, Y, Z, pfx, pins):
 def cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, budget_s):
     """The oracle (C++, OpenMP over the nets of a batch, batches in order) on
     the host cores: whole passes on all threads, the median of up to 5 that
-    fit in the budget; then one thread over the batches that fit in a third
-    of the budget, with the all-thread rate over the same batches beside it."""
+    fit in 80% of the budget; then one thread over the batches that fit in a
+    fifth of the budget, with the all-thread rate over the same batches
+    beside it."""
     import oracle
     threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count(), os.cpu_count())
     nb = len(ofst) - 1
@@ -75,16 +76,16 @@ def cpu_baseline(X, Y, Z, cap, pfx, pins, ofst, ids, budget_s):
         return per
 
     # all threads: whole passes (the first one cut at half the budget: then
-    # the batches it reached are the sample), up to 5 within the budget
+    # the batches it reached are the sample), up to 5 within 80% of the budget
     runs = [one_pass(threads, stop_s=budget_s / 2)]
     nb_s = len(runs[0])
     pass_s = sum(runs[0])
-    while nb_s == nb and len(runs) < 5 and sum(sum(r) for r in runs) + pass_s < budget_s * 2 / 3:
+    while nb_s == nb and len(runs) < 5 and sum(sum(r) for r in runs) + pass_s < budget_s * 0.8:
         runs.append(one_pass(threads))
     times = sorted(sum(r) for r in runs)
     med = times[len(times) // 2]
     nets_s = int(ofst[nb_s])
-    single = one_pass(1, stop_s=budget_s / 3)[:nb_s]
+    single = one_pass(1, stop_s=budget_s / 5)[:nb_s]
     k = len(single)
     nets_k = int(ofst[k])
     multi_k = sorted(sum(r[:k]) for r in runs)[len(runs) // 2]
@@ -356,7 +357,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--cpu-seconds", type=float, default=50.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
     ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
