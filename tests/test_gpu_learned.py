@@ -291,6 +291,59 @@ def test_always_fallback_full_workload(gr, gpu):
     r.set_proposal(None)
 
 
+def _sampled_topk_check(r, m, o, cap, pfx, pins, ofst, ids, b, k, rng, n_sample=3000, thr=0.6):
+    """Route batch b (the router holds the state before it, synced) and check
+    sampled pairs against the oracle's costing at that state: the (cost, p)
+    minimum over the model's top-k candidates when the top-1 softmax
+    probability is at least thr, else the exhaustive minimum.  Returns the
+    number of sampled pairs of each kind (top-k, fell back)."""
+    Z2 = (r.Z + 1) // 2
+    o.set_rem(cap - r.demand())
+    batch = ids[ofst[b]:ofst[b + 1]]
+    deg = np.diff(pfx)[batch]
+    P = deg * (deg - 1) // 2
+    first = np.concatenate([[0], np.cumsum(P)])
+    tot = int(first[-1])
+    sample = np.sort(rng.choice(tot, size=min(tot, n_sample), replace=False))
+    owner = np.searchsorted(first, sample, side="right") - 1
+    src, dst = [], []
+    for w, q in zip(sample, owner):
+        o_ = int(w - first[q])
+        j = int((1 + np.sqrt(1 + 8 * o_)) // 2)
+        while j * (j - 1) // 2 > o_:
+            j -= 1
+        while (j + 1) * j // 2 <= o_:
+            j += 1
+        kk = o_ - j * (j - 1) // 2
+        p = pins[pfx[batch[q]]:pfx[batch[q] + 1]]
+        src.append(int(p[j])); dst.append(int(p[kk]))
+    logits = m.forward(m.features(np.array(src, np.int32), np.array(dst, np.int32)))[:, :m.n_out]
+    r.route_schedule(b, 1)
+    cost, pat = r.pair_costs()
+    seen_topk = seen_fb = 0
+    for i, w in enumerate(sample):
+        lg = logits[i].astype(np.float64)
+        p1 = 1.0 / np.exp(lg - lg.max()).sum()
+        order = sorted(range(m.n_out), key=lambda q: (-logits[i, q], q))[:k]
+        top = None
+        for q in order:
+            c = o.pattern_cost(src[i], dst[i], q + Z2)
+            if c is not None and (top is None or (np.float32(c), q + Z2) < top):
+                top = (np.float32(c), q + Z2)
+        pp, cc = o.best_pattern(src[i], dst[i])
+        full = (np.float32(cc), pp)
+        got = (np.float32(cost[w]), int(pat[w]))
+        if abs(p1 - thr) < 1e-4:
+            assert got in (top, full), (k, b, i)
+        elif p1 < thr or top is None:
+            assert got == full, (k, b, i, got, full)
+            seen_fb += 1
+        else:
+            assert got == top, (k, b, i, got, top)
+            seen_topk += 1
+    return seen_topk, seen_fb
+
+
 def test_trained_topk_full_workload_sampled(gr, orc, gpu):
     """The learned workload at its full size: 800x800x10, 150,000 nets
     requested, a model trained on exhaustive routing of two other seeds,
@@ -301,7 +354,6 @@ def test_trained_topk_full_workload_sampled(gr, orc, gpu):
     probability is at least 0.6, else the exhaustive minimum.  Both kinds of
     pair must occur."""
     X, Y, Z = 800, 800, 10
-    Z2 = (Z + 1) // 2
     cap = gr.gen_grid(2, 0, X, Y, Z)
     r = gr.Router(X, Y, Z, cap)
     m = gr.Model(r, hidden=256, seed=0, max_samples=1 << 18)
@@ -333,49 +385,71 @@ def test_trained_topk_full_workload_sampled(gr, orc, gpu):
             if b:
                 r.route_schedule(0, b)
             r.sync()
-            o.set_rem(cap - r.demand())
-            batch = ids[ofst[b]:ofst[b + 1]]
-            deg = np.diff(pfx)[batch]
-            P = deg * (deg - 1) // 2
-            first = np.concatenate([[0], np.cumsum(P)])
-            tot = int(first[-1])
-            sample = np.sort(rng.choice(tot, size=min(tot, 3000), replace=False))
-            owner = np.searchsorted(first, sample, side="right") - 1
-            src, dst = [], []
-            for w, q in zip(sample, owner):
-                o_ = int(w - first[q])
-                j = int((1 + np.sqrt(1 + 8 * o_)) // 2)
-                while j * (j - 1) // 2 > o_:
-                    j -= 1
-                while (j + 1) * j // 2 <= o_:
-                    j += 1
-                kk = o_ - j * (j - 1) // 2
-                p = pins[pfx[batch[q]]:pfx[batch[q] + 1]]
-                src.append(int(p[j])); dst.append(int(p[kk]))
-            logits = m.forward(m.features(np.array(src, np.int32), np.array(dst, np.int32)))[:, :m.n_out]
-            r.route_schedule(b, 1)
-            cost, pat = r.pair_costs()
-            for i, w in enumerate(sample):
-                lg = logits[i].astype(np.float64)
-                p1 = 1.0 / np.exp(lg - lg.max()).sum()
-                order = sorted(range(m.n_out), key=lambda q: (-logits[i, q], q))[:k]
-                top = None
-                for q in order:
-                    c = o.pattern_cost(src[i], dst[i], q + Z2)
-                    if c is not None and (top is None or (np.float32(c), q + Z2) < top):
-                        top = (np.float32(c), q + Z2)
-                pp, cc = o.best_pattern(src[i], dst[i])
-                full = (np.float32(cc), pp)
-                got = (np.float32(cost[w]), int(pat[w]))
-                if abs(p1 - 0.6) < 1e-4:
-                    assert got in (top, full), (k, b, i)
-                elif p1 < 0.6 or top is None:
-                    assert got == full, (k, b, i, got, full)
-                    seen_fb += 1
-                else:
-                    assert got == top, (k, b, i, got, top)
-                    seen_topk += 1
+            t, f = _sampled_topk_check(r, m, o, cap, pfx, pins, ofst, ids, b, k, rng)
+            seen_topk += t
+            seen_fb += f
     r.set_proposal(None)
     print(f"sampled pairs: {seen_topk} from top-k, {seen_fb} fell back", flush=True)
     assert seen_fb > 0 and seen_topk > 0
     m.close()
+
+
+def test_stress_online_finetune_sampled(gr, orc, gpu):
+    """The combined-stress workload at its full size (2000x2000x10,
+    capacities cut 40% plus 30 hotspots, 1.5M nets requested) with the
+    online fine-tuning of the learned pass: a model trained on exhaustive
+    routing of another seed routes batches 0..9 with k = 4 and threshold
+    0.6 while it captures fallback-judged samples, is fine-tuned on them
+    for one epoch (as the learned pass does every 50 batches), and then
+    routes batches 10 and 11.  The fine-tuning must change the model, and
+    on sampled pairs of batches 10 and 11 (bulk batches, where both kinds
+    of pair occur; later batches fall back almost entirely) each
+    pair's (cost, pattern) equals the oracle's costing with the fine-tuned
+    model at the GPU's own state before the batch."""
+    W = gr.WORKLOADS[5]
+    X, Y, Z = W["X"], W["Y"], W["Z"]
+    cap = gr.gen_grid(5, 0, X, Y, Z)
+    r = gr.Router(X, Y, Z, cap)
+    m = gr.Model(r, hidden=256, seed=0, max_samples=1 << 20)
+    m.collect(500)
+    pf, pn = gr.gen_nets(5, 1, X, Y, Z, W["nets"])
+    of, ii = gr.partition_nets(X, Y, Z, pf, pn, 4)
+    r.set_nets(pf, pn)
+    r.set_schedule(of, ii)
+    r.reset_demand()
+    r.route_schedule()
+    r.sync()
+    m.collect(0)
+    for e in range(2):
+        m.train(epochs=1, batch=1024, lr=0.05, seed=e)
+    m.clear_samples()
+    pfx, pins = gr.gen_nets(5, 0, X, Y, Z, W["nets"])
+    ofst, ids = gr.partition_nets(X, Y, Z, pfx, pins, 4)
+    assert len(ofst) - 1 > 12
+    r.set_nets(pfx, pins)
+    r.set_schedule(ofst, ids)
+    r.set_proposal(m, k=4, threshold=0.6)
+    m.collect(500)
+    r.reset_demand()
+    print("stress: batches 0..9 with sample capture", flush=True)
+    r.route_schedule(0, 10)
+    r.sync()
+    before = m.params().copy()
+    m.train(epochs=1, batch=1024, lr=0.02, seed=1000)
+    m.clear_samples()
+    m.collect(0)
+    assert not np.array_equal(m.params(), before), "fine-tuning left the model unchanged"
+    o = orc.Oracle(X, Y, Z, cap)
+    rng = np.random.default_rng(11)
+    seen_topk = seen_fb = 0
+    for b in (10, 11):
+        print(f"stress fine-tuned, batch {b}", flush=True)
+        r.sync()
+        t, f = _sampled_topk_check(r, m, o, cap, pfx, pins, ofst, ids, b, 4, rng, n_sample=6000)
+        seen_topk += t
+        seen_fb += f
+    r.set_proposal(None)
+    print(f"sampled pairs: {seen_topk} from top-k, {seen_fb} fell back", flush=True)
+    assert seen_fb > 0 and seen_topk > 0
+    m.close()
+    r.close()
