@@ -590,6 +590,164 @@ struct PropArgs {
   uint8_t* fbout;  // per pair: fell back to every pattern (sample capture), or null
 };
 
+// Hidden layers of one tile: act holds the tile's inputs (features) on entry
+// and H2 on return; a0 holds W1's first K-step fragments of this wave
+// (mma_first), loaded ahead by the caller.  Wave w owns neurons
+// [16 NFH w, 16 NFH (w + 1)) of both layers for all TT rows.
+template <int ZT, int TT, int NW>
+__device__ __forceinline__ void prop_hidden(const PropArgs& A, uint16_t* act, const uint4 (&a0)[16 / NW], int wv,
+                                            int lane) {
+  constexpr int NFH = 16 / NW;
+  using S = PropShape<ZT>;
+  constexpr int RS = S::RS, KIN = S::KIN;
+  const uint16_t* w1 = A.W1p + (size_t)wv * NFH * (KIN / 32) * 512;
+  const uint16_t* w2 = A.W2p + (size_t)wv * NFH * (kHid / 32) * 512;
+  v4f acc[NFH][TT / 16];
+  mma_tile<KIN / 32, NFH, TT / 16>(w1, a0, act, RS, 0, acc, lane);
+  uint4 a2[NFH];
+  mma_first<kHid / 32, NFH>(w2, a2, lane);
+  __syncthreads();
+  store_hidden<NFH, TT / 16>(acc, A.b1, wv * 16 * NFH, act, RS, lane);
+  __syncthreads();
+  mma_tile<kHid / 32, NFH, TT / 16>(w2, a2, act, RS, 0, acc, lane);
+  __syncthreads();
+  store_hidden<NFH, TT / 16>(acc, A.b2, wv * 16 * NFH, act, RS, lane);
+  __syncthreads();
+}
+
+// Output layer and top-k of one routed tile (act holds H2): wave f < NF3
+// computes output neurons [16f, 16f + 16) for all TT pairs, so each W3
+// fragment is loaded once per tile; the fp32 logits go to LDS over the
+// activations, then each pair's 4 lanes take k rounds of the best (value
+// desc, index asc) logit not yet taken, and the softmax top-1 probability
+// comes from the 4-lane maximum and sum.  Writes smask and sfb for the
+// tile's TT pairs; ends with a barrier.
+template <int ZT, int TT, int NW>
+__device__ __forceinline__ void prop_out_topk(const PropArgs& A, uint16_t* act, unsigned long long* smask,
+                                              int8_t* sfb, int wv, int lane) {
+  using S = PropShape<ZT>;
+  constexpr int NF3 = S::NF3, NOUT = S::NOUT;
+  constexpr int QW = TT / 16 / NW;
+  static_assert(NF3 <= NW, "one output fragment per wave");
+  const bool has3 = wv < NF3;  // wave-uniform
+  v4f acc[1][TT / 16];
+  float4 b3v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (has3) {
+    const uint16_t* w3 = A.W3p + (size_t)wv * (kHid / 32) * 512;
+    uint4 a3[1];
+    mma_first<kHid / 32, 1>(w3, a3, lane);
+    mma_tile<kHid / 32, 1, TT / 16>(w3, a3, act, S::RS, 0, acc, lane);
+    b3v = *reinterpret_cast<const float4*>(A.b3 + 16 * wv + 4 * (lane >> 4));
+  }
+  // logits (fp32) into LDS over the activations, once every wave is done with them
+  constexpr int LS = NF3 * 16 + 4;  // row stride in floats
+  static_assert(TT * LS * 4 <= TT * S::RS * 2, "logits fit the activation buffer");
+  float* lg = reinterpret_cast<float*>(act);
+  __syncthreads();
+  if (has3) {
+    const int n = 16 * wv + 4 * (lane >> 4);
+#pragma unroll
+    for (int q = 0; q < TT / 16; ++q)
+      *reinterpret_cast<float4*>(lg + (16 * q + (lane & 15)) * LS + n) =
+          make_float4(acc[0][q][0] + b3v.x, acc[0][q][1] + b3v.y, acc[0][q][2] + b3v.z, acc[0][q][3] + b3v.w);
+  }
+  __syncthreads();
+  const int gq = lane >> 4;
+#pragma unroll
+  for (int q = 0; q < QW; ++q) {
+    float vals[NF3][4];
+    float mxl = -INFINITY;
+    const int prq = 16 * QW * wv + 16 * q + (lane & 15);
+#pragma unroll
+    for (int f = 0; f < NF3; ++f) {
+      const int nb = 16 * f + 4 * gq;
+      const float4 lv4 = *reinterpret_cast<const float4*>(lg + prq * LS + nb);
+      vals[f][0] = lv4.x;
+      vals[f][1] = lv4.y;
+      vals[f][2] = lv4.z;
+      vals[f][3] = lv4.w;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (nb + i < NOUT) mxl = fmaxf(mxl, vals[f][i]);
+    }
+    float mx = fmaxf(mxl, __shfl_xor(mxl, 16, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    float sl = 0.f;
+#pragma unroll
+    for (int f = 0; f < NF3; ++f)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (16 * f + 4 * gq + i < NOUT) sl += expf(vals[f][i] - mx);
+    float sm = sl + __shfl_xor(sl, 16, 64);
+    sm = sm + __shfl_xor(sm, 32, 64);
+    uint32_t taken = 0;  // bit f*4 + i: this lane's logit already chosen
+    uint64_t msk = 0;
+    for (int t = 0; t < A.topk; ++t) {
+      float bv = -INFINITY;
+      int bi = 127;  // none
+#pragma unroll
+      for (int f = 0; f < NF3; ++f)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int n = 16 * f + 4 * gq + i;
+          // ascending n, strict >: the lowest index wins among equal values
+          if (n < NOUT && !((taken >> (f * 4 + i)) & 1) && (vals[f][i] > bv || bi == 127)) {
+            bv = vals[f][i];
+            bi = n;
+          }
+        }
+#pragma unroll
+      for (int d = 16; d <= 32; d <<= 1) {
+        const float ov = __shfl_xor(bv, d, 64);
+        const int oi = __shfl_xor(bi, d, 64);
+        if (oi != 127 && (bi == 127 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+      }
+      if (bi < NOUT) {
+        msk |= 1ull << bi;
+        if (((bi >> 2) & 3) == gq) taken |= 1u << ((bi >> 4) * 4 + (bi & 3));  // this lane's logit
+      }
+    }
+    if (gq == 0) {
+      smask[prq] = msk;
+      sfb[prq] = (int8_t)((1.0f / sm) < A.thr);
+    }
+  }
+  __syncthreads();
+}
+
+// Candidate costing of pair pr of a routed tile for one parity of the first
+// wire layer (uniform per wave, so the loads do not diverge by parity); only
+// the terms of the pair's candidates are loaded.  Waves 2h and 2h + 1 take
+// pairs 64h .. 64h + 63; the two parity keys meet in skey[2][TT].
+template <int ZT, int TT>
+__device__ __forceinline__ void prop_cost_pair(const PropArgs& A, int nrow, int pr, int par, const int16_t (*sc)[6],
+                                               const unsigned long long* smask, const int8_t* sfb,
+                                               unsigned long long* skey) {
+  const int pc = pr < nrow ? pr : nrow - 1;
+  const int z1 = sc[pc][0], x1 = sc[pc][1], y1 = sc[pc][2], z2 = sc[pc][3], x2 = sc[pc][4], y2 = sc[pc][5];
+  skey[par * TT + pr] =
+      lane_best<ZT, true>(A.g, A.pfx, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0, A.vterm);
+}
+
+// Result of pair pr of the tile at t0: the smaller of its two parity keys;
+// no valid candidate -> every pattern.  Returns whether the pair fell back.
+template <int ZT, int TT>
+__device__ __forceinline__ bool prop_finish_pair(const PropArgs& A, int64_t t0, int pr, const int16_t (*sc)[6],
+                                                 const int8_t* sfb, const unsigned long long* skey) {
+  const uint64_t k0 = skey[pr], k1 = skey[TT + pr];
+  uint64_t best = k0 < k1 ? k0 : k1;
+  bool fb = sfb[pr] != 0;
+  if (best == ~0ull) {
+    fb = true;
+    best = lane_best<ZT>(A.g, A.pfx, sc[pr][0], sc[pr][1], sc[pr][2], sc[pr][3], sc[pr][4], sc[pr][5], ~0ull, 0, true,
+                         A.vterm);
+  }
+  A.out_cost[t0 + pr] = key_cost((uint32_t)(best >> 32));
+  A.out_pat[t0 + pr] = (int16_t)(uint32_t)best;
+  if (A.fbout) A.fbout[t0 + pr] = fb ? 1 : 0;
+  return fb;
+}
+
 // ROUTE: features of the batch's pairs gathered from the grid, top-k selection
 // from the output fragments, then candidate costing with the full-enumeration
 // fallback (k_lpat_topk's rules).  !ROUTE: rows of A.X in, logits out (the
@@ -598,9 +756,8 @@ template <int ZT, bool ROUTE, int TT, int NW = TT / 16>
 __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
   constexpr int NT = 64 * NW;
   constexpr int NFH = 16 / NW;
-  constexpr int QW = TT / 16 / NW;
   using S = PropShape<ZT>;
-  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, NF3 = S::NF3, NOUT = S::NOUT;
+  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT;
   __shared__ __attribute__((aligned(16))) uint16_t act[TT * RS];
   __shared__ uint16_t slut[ROUTE ? kLut : 1];         // log-cost table as bf16
   __shared__ int16_t sx[ROUTE ? TT : 1][5], sy[ROUTE ? TT : 1][5];  // lattice columns / rows
@@ -612,10 +769,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
   const int64_t t0 = (int64_t)blockIdx.x * TT;
   const int nrow = (int)(A.P - t0 < TT ? A.P - t0 : TT);
   const Geo& g = A.g;
-  const uint16_t* w1 = A.W1p + (size_t)wv * NFH * (KIN / 32) * 512;
-  const uint16_t* w2 = A.W2p + (size_t)wv * NFH * (kHid / 32) * 512;
   uint4 a0[NFH];
-  mma_first<KIN / 32, NFH>(w1, a0, lane);
+  mma_first<KIN / 32, NFH>(A.W1p + (size_t)wv * NFH * (KIN / 32) * 512, a0, lane);
 
   if (ROUTE) {
     if (blockIdx.x == 0 && tid == 0) *A.ctr += 1;  // this batch's stamp
@@ -698,38 +853,19 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
   }
   __syncthreads();
 
-  // hidden layers
-  {
-    v4f acc[NFH][TT / 16];
-    mma_tile<KIN / 32, NFH, TT / 16>(w1, a0, act, RS, 0, acc, lane);
-    mma_first<kHid / 32, NFH>(w2, a0, lane);
-    __syncthreads();
-    store_hidden<NFH, TT / 16>(acc, A.b1, wv * 16 * NFH, act, RS, lane);
-    __syncthreads();
-    mma_tile<kHid / 32, NFH, TT / 16>(w2, a0, act, RS, 0, acc, lane);
-    __syncthreads();
-    store_hidden<NFH, TT / 16>(acc, A.b2, wv * 16 * NFH, act, RS, lane);
-    __syncthreads();
-  }
-
-  // output layer, split by neuron fragment: wave f < NFL computes output
-  // neurons [16f, 16f + 16) for all TT pairs, so each W3 fragment is loaded
-  // once per tile; lane l then holds, for pair 16q + (l & 15), the logits of
-  // neurons 16f + 4(l >> 4) + i (i < 4)
-  constexpr int NFL = ROUTE ? NF3 : S::NFO;
-  static_assert(NFL <= NW, "one output fragment per wave");
-  const bool has3 = wv < NFL;  // wave-uniform
-  v4f acc[1][TT / 16];
-  float4 b3v = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (has3) {
-    const uint16_t* w3 = A.W3p + (size_t)wv * (kHid / 32) * 512;
-    uint4 a3[1];
-    mma_first<kHid / 32, 1>(w3, a3, lane);
-    mma_tile<kHid / 32, 1, TT / 16>(w3, a3, act, RS, 0, acc, lane);
-    b3v = *reinterpret_cast<const float4*>(A.b3 + 16 * wv + 4 * (lane >> 4));
-  }
+  prop_hidden<ZT, TT, NW>(A, act, a0, wv, lane);
   if (!ROUTE) {
-    if (has3) {
+    // output layer, split by neuron fragment: wave f < NFO computes output
+    // neurons [16f, 16f + 16) for all TT rows; lane l then holds, for row
+    // 16q + (l & 15), the logits of neurons 16f + 4(l >> 4) + i (i < 4)
+    static_assert(S::NFO <= NW, "one output fragment per wave");
+    if (wv < S::NFO) {
+      v4f acc[1][TT / 16];
+      const uint16_t* w3 = A.W3p + (size_t)wv * (kHid / 32) * 512;
+      uint4 a3[1];
+      mma_first<kHid / 32, 1>(w3, a3, lane);
+      mma_tile<kHid / 32, 1, TT / 16>(w3, a3, act, RS, 0, acc, lane);
+      const float4 b3v = *reinterpret_cast<const float4*>(A.b3 + 16 * wv + 4 * (lane >> 4));
       const int n = 16 * wv + 4 * (lane >> 4);
 #pragma unroll
       for (int q = 0; q < TT / 16; ++q) {
@@ -741,116 +877,140 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
     }
     return;
   }
-  // logits (fp32) into LDS over the activations, once every wave is done with them
-  constexpr int LS = NF3 * 16 + 4;  // row stride in floats
-  static_assert(TT * LS * 4 <= TT * RS * 2, "logits fit the activation buffer");
-  float* lg = reinterpret_cast<float*>(act);
-  __syncthreads();
-  if (has3) {
-    const int n = 16 * wv + 4 * (lane >> 4);
-#pragma unroll
-    for (int q = 0; q < TT / 16; ++q)
-      *reinterpret_cast<float4*>(lg + (16 * q + (lane & 15)) * LS + n) =
-          make_float4(acc[0][q][0] + b3v.x, acc[0][q][1] + b3v.y, acc[0][q][2] + b3v.z, acc[0][q][3] + b3v.w);
-  }
-  __syncthreads();
-
-  // top-k per pair over its 4 lanes: k rounds, each taking the best (value
-  // desc, index asc) logit not yet taken; softmax top-1 probability from the
-  // 4-lane max and sum
-  const int gq = lane >> 4;
-#pragma unroll
-  for (int q = 0; q < QW; ++q) {
-    float vals[NF3][4];
-    float mxl = -INFINITY;
-    const int prq = 16 * QW * wv + 16 * q + (lane & 15);
-#pragma unroll
-    for (int f = 0; f < NF3; ++f) {
-      const int nb = 16 * f + 4 * gq;
-      const float4 lv4 = *reinterpret_cast<const float4*>(lg + prq * LS + nb);
-      vals[f][0] = lv4.x;
-      vals[f][1] = lv4.y;
-      vals[f][2] = lv4.z;
-      vals[f][3] = lv4.w;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (nb + i < NOUT) mxl = fmaxf(mxl, vals[f][i]);
-    }
-    float mx = fmaxf(mxl, __shfl_xor(mxl, 16, 64));
-    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-    float sl = 0.f;
-#pragma unroll
-    for (int f = 0; f < NF3; ++f)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (16 * f + 4 * gq + i < NOUT) sl += expf(vals[f][i] - mx);
-    float sm = sl + __shfl_xor(sl, 16, 64);
-    sm = sm + __shfl_xor(sm, 32, 64);
-    uint32_t taken = 0;  // bit f*4 + i: this lane's logit already chosen
-    uint64_t msk = 0;
-    for (int t = 0; t < A.topk; ++t) {
-      float bv = -INFINITY;
-      int bi = 127;  // none
-#pragma unroll
-      for (int f = 0; f < NF3; ++f)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int n = 16 * f + 4 * gq + i;
-          // ascending n, strict >: the lowest index wins among equal values
-          if (n < NOUT && !((taken >> (f * 4 + i)) & 1) && (vals[f][i] > bv || bi == 127)) {
-            bv = vals[f][i];
-            bi = n;
-          }
-        }
-#pragma unroll
-      for (int d = 16; d <= 32; d <<= 1) {
-        const float ov = __shfl_xor(bv, d, 64);
-        const int oi = __shfl_xor(bi, d, 64);
-        if (oi != 127 && (bi == 127 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
-      }
-      if (bi < NOUT) {
-        msk |= 1ull << bi;
-        if (((bi >> 2) & 3) == gq) taken |= 1u << ((bi >> 4) * 4 + (bi & 3));  // this lane's logit
-      }
-    }
-    if (gq == 0) {
-      const int pr = 16 * QW * wv + 16 * q + (lane & 15);
-      smask[pr] = msk;
-      sfb[pr] = (int8_t)((1.0f / sm) < A.thr);
-    }
-  }
-  __syncthreads();
-
-  // candidate costing: waves 2h and 2h + 1 take pairs 64h .. 64h + 63, one
-  // parity of the first wire layer each (uniform per wave, so the loads do not
-  // diverge by parity); only the terms of the lane's candidates are loaded.
-  // The two keys meet in LDS; no valid candidate -> every pattern.
+  prop_out_topk<ZT, TT, NW>(A, act, smask, sfb, wv, lane);
   unsigned long long* skey = reinterpret_cast<unsigned long long*>(act);  // [2][TT]
-  if (64 * (wv >> 1) < TT) {
-    const int pr = 64 * (wv >> 1) + lane, par = wv & 1;
-    const int pc = pr < nrow ? pr : nrow - 1;
-    const int z1 = sc[pc][0], x1 = sc[pc][1], y1 = sc[pc][2], z2 = sc[pc][3], x2 = sc[pc][4], y2 = sc[pc][5];
-    skey[par * TT + pr] =
-        lane_best<ZT, true>(g, A.pfx, z1, x1, y1, z2, x2, y2, smask[pc], par, sfb[pc] != 0, A.vterm);
-  }
+  if (64 * (wv >> 1) < TT) prop_cost_pair<ZT, TT>(A, nrow, 64 * (wv >> 1) + lane, wv & 1, sc, smask, sfb, skey);
   __syncthreads();
-  if (tid < nrow) {
-    const uint64_t k0 = skey[tid], k1 = skey[TT + tid];
-    uint64_t best = k0 < k1 ? k0 : k1;
-    bool fb = sfb[tid] != 0;
-    if (best == ~0ull) {
-      fb = true;
-      best = lane_best<ZT>(g, A.pfx, sc[tid][0], sc[tid][1], sc[tid][2], sc[tid][3], sc[tid][4], sc[tid][5], ~0ull, 0,
-                           true, A.vterm);
-    }
-    A.out_cost[t0 + tid] = key_cost((uint32_t)(best >> 32));
-    A.out_pat[t0 + tid] = (int16_t)(uint32_t)best;
-    if (A.fbout) A.fbout[t0 + tid] = fb ? 1 : 0;
-    if (fb) atomicAdd(&nfb, 1);
-  }
+  if (tid < nrow && prop_finish_pair<ZT, TT>(A, t0, tid, sc, sfb, skey)) atomicAdd(&nfb, 1);
   __syncthreads();
   if (tid == 0) {
     atomicAdd(&A.stats[0], (unsigned long long)nrow);
+    atomicAdd(&A.stats[1], (unsigned long long)nfb);
+  }
+}
+
+// Features of one routed tile (pairs t0 .. t0 + nrow - 1, from the endpoint
+// table) by TT * tpp threads: thread j works on pair j / tpp, decoding it
+// itself, and gathers its lattice points uv = j % tpp, + tpp, ... < 25, all
+// ZT layers of each from the per-GCELL layer copy (feature 4 + l*25 + uv,
+// uv = 5u + v for lattice column u and row v).  No thread reads what
+// another wrote, so no barrier separates the decode from the gather.  The
+// first thread of each pair writes its coordinates (sc) and the row's
+// non-lattice features.
+template <int ZT, int TT>
+__device__ __forceinline__ void prop_tile_features(const PropArgs& A, int64_t t0, int nrow, int j, int tpp,
+                                                   uint16_t* act, const uint16_t* slut, int16_t (*sc)[6]) {
+  using S = PropShape<ZT>;
+  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, UG = 7;
+  const int pr = j / tpp, h = j - pr * tpp;
+  const uint2 e = A.ends[t0 + (pr < nrow ? pr : nrow - 1)];
+  int z1, x1, y1, z2, x2, y2;
+  unpack_cell(e.x, z1, x1, y1);
+  unpack_cell(e.y, z2, x2, y2);
+  const int dx = x2 - x1, dy = y2 - y1;
+  uint16_t* row = act + pr * RS;
+  if (h == 0) {
+    sc[pr][0] = (int16_t)z1; sc[pr][1] = (int16_t)x1; sc[pr][2] = (int16_t)y1;
+    sc[pr][3] = (int16_t)z2; sc[pr][4] = (int16_t)x2; sc[pr][5] = (int16_t)y2;
+    row[0] = f2bf((float)dx * (1.0f / 64.0f));
+    row[1] = f2bf((float)dy * (1.0f / 64.0f));
+    row[2] = f2bf((float)z1);
+    row[3] = f2bf((float)z2);
+    for (int f = 4 + NFEAT; f < KIN; ++f) row[f] = 0;
+  }
+#pragma unroll 1
+  for (int uv0 = h; uv0 < 25; uv0 += tpp * UG) {  // UG points in flight per step
+    uint32_t xy[UG];
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      const int uv = uv0 + tpp * u < 25 ? uv0 + tpp * u : 24, iu = uv / 5, iv = uv - 5 * iu;
+      const int cx = x1 + (dx * iu + (dx >= 0 ? 2 : -2)) / 4, cy = y1 + (dy * iv + (dy >= 0 ? 2 : -2)) / 4;
+      xy[u] = (uint32_t)cy * (uint32_t)A.g.X + (uint32_t)cx;
+    }
+    uint4 q[UG][2];
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      const uint4* src = reinterpret_cast<const uint4*>(A.remz + (size_t)xy[u] * 16);
+      q[u][0] = src[0];
+      if (ZT > 8) q[u][1] = src[1];
+    }
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      const int uv = uv0 + tpp * u;
+      if (uv >= 25) continue;
+      const uint32_t w[8] = {q[u][0].x, q[u][0].y, q[u][0].z, q[u][0].w,
+                             ZT > 8 ? q[u][1].x : 0u, ZT > 8 ? q[u][1].y : 0u, ZT > 8 ? q[u][1].z : 0u,
+                             ZT > 8 ? q[u][1].w : 0u};
+#pragma unroll
+      for (int l = 0; l < ZT; ++l) {
+        const int v = (int)(int16_t)(uint16_t)(w[l >> 1] >> (16 * (l & 1)));
+        row[4 + uv + 25 * l] = slut[v - kRMin];
+      }
+    }
+  }
+}
+
+// The routing kernel as a persistent loop over tiles t, t + grid, ... of a
+// batch with the endpoint table: while waves 0..3 cost tile t's candidates,
+// waves 4..7 decode the next tile's pairs and gather its features into the
+// activation buffer, which the costing does not touch (its keys have their
+// own LDS array, the coordinates two buffers).  Per tile the phases are the
+// layers and the top-k (all waves), then costing beside the next tile's
+// features, then the results; each tile's arithmetic is k_prop's.
+template <int ZT, int TT, int NW = TT / 16>
+__global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
+  constexpr int NT = 64 * NW;
+  constexpr int NFH = 16 / NW;
+  using S = PropShape<ZT>;
+  constexpr int RS = S::RS, KIN = S::KIN;
+  static_assert(TT == 128 && NW == 8, "four costing and four feature wavefronts");
+  __shared__ __attribute__((aligned(16))) uint16_t act[TT * RS];
+  __shared__ uint16_t slut[kLut];                  // log-cost table as bf16
+  __shared__ int16_t sc[2][TT][6];                 // z1 x1 y1 z2 x2 y2, this tile's and the next's
+  __shared__ unsigned long long smask[TT];         // candidate patterns q = p - Z2
+  __shared__ int8_t sfb[TT];                       // low confidence: enumerate every pattern
+  __shared__ unsigned long long skey[2 * TT];      // the two parity keys of each pair
+  __shared__ int nfb;
+  const int tid = threadIdx.x, wv0 = tid >> 6, lane0 = tid & 63;
+  const int64_t ntile = (A.P + TT - 1) / TT;
+  if (blockIdx.x == 0 && tid == 0) *A.ctr += 1;  // this batch's stamp
+  if (tid == 0) nfb = 0;
+  for (int i = tid; i < kLut; i += NT) slut[i] = f2bf(A.loglut[i]);
+  __syncthreads();
+  int64_t t = blockIdx.x;
+  int cur = 0;
+  int64_t rows = 0;
+  uint4 a0[NFH];
+  mma_first<KIN / 32, NFH>(A.W1p + (size_t)wv0 * NFH * (KIN / 32) * 512, a0, lane0);
+  prop_tile_features<ZT, TT>(A, t * TT, (int)(A.P - t * TT < TT ? A.P - t * TT : TT), tid, NT / TT, act, slut, sc[0]);
+  __syncthreads();
+  for (;;) {
+    // the thread index as opaque per iteration: the per-lane LDS addresses
+    // of the layers are then formed inside the loop, not held across it
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
+    const int wv = __builtin_amdgcn_readfirstlane(tl >> 6), lane = tl & 63;  // wv: a scalar, so the phase branch is uniform
+    const int64_t t0 = t * TT, tn = t + gridDim.x;
+    const int nrow = (int)(A.P - t0 < TT ? A.P - t0 : TT);
+    rows += nrow;
+    prop_hidden<ZT, TT, NW>(A, act, a0, wv, lane);
+    prop_out_topk<ZT, TT, NW>(A, act, smask, sfb, wv, lane);
+    if (wv < 4) {
+      prop_cost_pair<ZT, TT>(A, nrow, 64 * (wv >> 1) + lane, wv & 1, sc[cur], smask, sfb, skey);
+    } else if (tn < ntile) {
+      const int64_t n0 = tn * TT;
+      prop_tile_features<ZT, TT>(A, n0, (int)(A.P - n0 < TT ? A.P - n0 : TT), tl - 4 * 64, 2, act, slut, sc[cur ^ 1]);
+    }
+    __syncthreads();
+    if (tl < nrow && prop_finish_pair<ZT, TT>(A, t0, tl, sc[cur], sfb, skey)) atomicAdd(&nfb, 1);
+    if (tn >= ntile) break;
+    mma_first<KIN / 32, NFH>(A.W1p + (size_t)wv * NFH * (KIN / 32) * 512, a0, lane);  // the next tile's
+    t = tn;
+    cur ^= 1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    atomicAdd(&A.stats[0], (unsigned long long)rows);
     atomicAdd(&A.stats[1], (unsigned long long)nfb);
   }
 }
@@ -874,6 +1034,21 @@ PropFn prop_kernel_t(int Z) {
 template <bool ROUTE>
 PropFn prop_kernel(int Z) { return prop_kernel_t<ROUTE, kTile>(Z); }
 
+PropFn prop_tiles_kernel(int Z) {
+  switch (Z) {
+    case 3: return k_prop_tiles<3, kTile>;
+    case 4: return k_prop_tiles<4, kTile>;
+    case 5: return k_prop_tiles<5, kTile>;
+    case 6: return k_prop_tiles<6, kTile>;
+    case 7: return k_prop_tiles<7, kTile>;
+    case 8: return k_prop_tiles<8, kTile>;
+    case 9: return k_prop_tiles<9, kTile>;
+    case 10: return k_prop_tiles<10, kTile>;
+    case 11: return k_prop_tiles<11, kTile>;
+    default: return nullptr;
+  }
+}
+
 int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
 }  // namespace
@@ -888,6 +1063,7 @@ struct gr_mlp {
   // fused inference (hidden width kHid): weights in MFMA fragment order
   uint16_t *W1p = nullptr, *W2p = nullptr, *W3p = nullptr;
   int nf3 = 0;  // packed output fragments of 16 neurons (n_op / 16)
+  int tiles_grid = 0;  // workgroups of the persistent routing kernel: as many as are resident at once
   int64_t cap_samples = 0, n_samples = 0;
   uint16_t* d_X = nullptr;
   int32_t* d_y = nullptr;
@@ -1055,7 +1231,12 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.ctr = c->d_ctr;
     A.stats = (unsigned long long*)c->d_prop_stats;
     A.fbout = c->collect ? c->d_pair_fb : nullptr;
-    hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(blocks_for(bm.P, kTile)), dim3(kTile * 4), 0, c->stream, A);
+    const int tiles = blocks_for(bm.P, kTile);
+    if (A.ends && m->tiles_grid > 0)
+      hipLaunchKernelGGL(prop_tiles_kernel(c->g.Z), dim3(std::min(tiles, m->tiles_grid)), dim3(kTile * 4), 0, c->stream,
+                         A);
+    else
+      hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(tiles), dim3(kTile * 4), 0, c->stream, A);
     HIPC(c, hipGetLastError());
     return 0;
   }
@@ -1135,6 +1316,11 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
     return bail(fail(c, GR_ENODEV, "model upload failed"));
   if (m->H == kHid && prop_kernel<true>(c->g.Z)) {
     m->nf3 = m->n_op / 16;  // every padded output column (the forward pass returns them all)
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, prop_tiles_kernel(c->g.Z), kTile * 4, 0) != hipSuccess)
+      return bail(fail(c, GR_ENODEV, "device query for the routing kernel failed"));
+    m->tiles_grid = ncu * per_cu;
     if ((rc = dalloc(c, &m->W1p, (size_t)m->k_in * m->H)) || (rc = dalloc(c, &m->W2p, (size_t)m->H * m->H)) ||
         (rc = dalloc(c, &m->W3p, (size_t)m->H * m->n_op)))
       return bail(rc);
