@@ -897,12 +897,39 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop(PropArgs A) {
 // another wrote, so no barrier separates the decode from the gather.  The
 // first thread of each pair writes its coordinates (sc) and the row's
 // non-lattice features.
-template <int ZT, int TT>
-__device__ __forceinline__ void prop_tile_features(const PropArgs& A, int64_t t0, int nrow, int j, int tpp,
-                                                   uint16_t* act, const uint16_t* slut, int16_t (*sc)[6]) {
+// The ZT layer values of one GCELL of the per-GCELL layer copy (16 int16
+// per cell, 32-byte aligned) as 32-bit words w[l / 2], loading only the
+// words that hold layers below ZT.
+template <int ZT>
+__device__ __forceinline__ void load_remz(const int16_t* cell, uint32_t (&w)[8]) {
+  const uint4 a = *reinterpret_cast<const uint4*>(cell);
+  w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+  w[4] = w[5] = w[6] = w[7] = 0u;
+  if constexpr (ZT > 12) {
+    const uint4 b = *reinterpret_cast<const uint4*>(cell + 8);
+    w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+  } else if constexpr (ZT > 10) {
+    const uint2 b = *reinterpret_cast<const uint2*>(cell + 8);
+    w[4] = b.x; w[5] = b.y;
+  } else if constexpr (ZT > 8) {
+    w[4] = *reinterpret_cast<const uint32_t*>(cell + 8);
+  }
+}
+
+// Features of one routed tile (pairs t0 .. t0 + nrow - 1, from the endpoint
+// table) by TT * TPP threads: thread j works on pair j / TPP, decoding it
+// itself, and gathers its lattice points uv = j % TPP, + TPP, ... < 25 (all
+// of them in flight at once), all ZT layers of each from the per-GCELL layer
+// copy (feature 4 + l*25 + uv, uv = 5u + v for lattice column u and row v).
+// No thread reads what another wrote, so no barrier separates the decode
+// from the gather.  The first thread of each pair writes its coordinates
+// (sc) and the row's non-lattice features.
+template <int ZT, int TT, int TPP>
+__device__ __forceinline__ void prop_tile_features(const PropArgs& A, int64_t t0, int nrow, int j, uint16_t* act,
+                                                   const uint16_t* slut, int16_t (*sc)[6]) {
   using S = PropShape<ZT>;
-  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, UG = 7;
-  const int pr = j / tpp, h = j - pr * tpp;
+  constexpr int RS = S::RS, KIN = S::KIN, NFEAT = S::NFEAT, UG = (25 + TPP - 1) / TPP;
+  const int pr = j / TPP, h = j - pr * TPP;
   const uint2 e = A.ends[t0 + (pr < nrow ? pr : nrow - 1)];
   int z1, x1, y1, z2, x2, y2;
   unpack_cell(e.x, z1, x1, y1);
@@ -918,34 +945,21 @@ __device__ __forceinline__ void prop_tile_features(const PropArgs& A, int64_t t0
     row[3] = f2bf((float)z2);
     for (int f = 4 + NFEAT; f < KIN; ++f) row[f] = 0;
   }
-#pragma unroll 1
-  for (int uv0 = h; uv0 < 25; uv0 += tpp * UG) {  // UG points in flight per step
-    uint32_t xy[UG];
+  uint32_t w[UG][8];
 #pragma unroll
-    for (int u = 0; u < UG; ++u) {
-      const int uv = uv0 + tpp * u < 25 ? uv0 + tpp * u : 24, iu = uv / 5, iv = uv - 5 * iu;
-      const int cx = x1 + (dx * iu + (dx >= 0 ? 2 : -2)) / 4, cy = y1 + (dy * iv + (dy >= 0 ? 2 : -2)) / 4;
-      xy[u] = (uint32_t)cy * (uint32_t)A.g.X + (uint32_t)cx;
-    }
-    uint4 q[UG][2];
+  for (int u = 0; u < UG; ++u) {
+    const int uv = h + TPP * u < 25 ? h + TPP * u : 24, iu = uv / 5, iv = uv - 5 * iu;
+    const int cx = x1 + (dx * iu + (dx >= 0 ? 2 : -2)) / 4, cy = y1 + (dy * iv + (dy >= 0 ? 2 : -2)) / 4;
+    load_remz<ZT>(A.remz + ((size_t)cy * (size_t)A.g.X + (size_t)cx) * 16, w[u]);
+  }
 #pragma unroll
-    for (int u = 0; u < UG; ++u) {
-      const uint4* src = reinterpret_cast<const uint4*>(A.remz + (size_t)xy[u] * 16);
-      q[u][0] = src[0];
-      if (ZT > 8) q[u][1] = src[1];
-    }
+  for (int u = 0; u < UG; ++u) {
+    const int uv = h + TPP * u;
+    if (uv >= 25) continue;
 #pragma unroll
-    for (int u = 0; u < UG; ++u) {
-      const int uv = uv0 + tpp * u;
-      if (uv >= 25) continue;
-      const uint32_t w[8] = {q[u][0].x, q[u][0].y, q[u][0].z, q[u][0].w,
-                             ZT > 8 ? q[u][1].x : 0u, ZT > 8 ? q[u][1].y : 0u, ZT > 8 ? q[u][1].z : 0u,
-                             ZT > 8 ? q[u][1].w : 0u};
-#pragma unroll
-      for (int l = 0; l < ZT; ++l) {
-        const int v = (int)(int16_t)(uint16_t)(w[l >> 1] >> (16 * (l & 1)));
-        row[4 + uv + 25 * l] = slut[v - kRMin];
-      }
+    for (int l = 0; l < ZT; ++l) {
+      const int v = (int)(int16_t)(uint16_t)(w[u][l >> 1] >> (16 * (l & 1)));
+      row[4 + uv + 25 * l] = slut[v - kRMin];
     }
   }
 }
@@ -982,7 +996,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
   int64_t rows = 0;
   uint4 a0[NFH];
   mma_first<KIN / 32, NFH>(A.W1p + (size_t)wv0 * NFH * (KIN / 32) * 512, a0, lane0);
-  prop_tile_features<ZT, TT>(A, t * TT, (int)(A.P - t * TT < TT ? A.P - t * TT : TT), tid, NT / TT, act, slut, sc[0]);
+  prop_tile_features<ZT, TT, NT / TT>(A, t * TT, (int)(A.P - t * TT < TT ? A.P - t * TT : TT), tid, act, slut, sc[0]);
   __syncthreads();
   for (;;) {
     // the thread index as opaque per iteration: the per-lane LDS addresses
@@ -999,7 +1013,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
       prop_cost_pair<ZT, TT>(A, nrow, 64 * (wv >> 1) + lane, wv & 1, sc[cur], smask, sfb, skey);
     } else if (tn < ntile) {
       const int64_t n0 = tn * TT;
-      prop_tile_features<ZT, TT>(A, n0, (int)(A.P - n0 < TT ? A.P - n0 : TT), tl - 4 * 64, 2, act, slut, sc[cur ^ 1]);
+      prop_tile_features<ZT, TT, 2>(A, n0, (int)(A.P - n0 < TT ? A.P - n0 : TT), tl - 4 * 64, act, slut, sc[cur ^ 1]);
     }
     __syncthreads();
     if (tl < nrow && prop_finish_pair<ZT, TT>(A, t0, tl, sc[cur], sfb, skey)) atomicAdd(&nfb, 1);
