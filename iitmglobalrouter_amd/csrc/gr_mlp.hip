@@ -977,7 +977,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
   constexpr int NFH = 16 / NW;
   using S = PropShape<ZT>;
   constexpr int RS = S::RS, KIN = S::KIN;
-  static_assert(TT == 128 && NW == 8, "four costing and four feature wavefronts");
+  constexpr int NC = TT / 32;  // costing wavefronts: (pair half, parity) each
+  static_assert(NW == TT / 16 && 2 * NC == NW, "as many costing as feature wavefronts");
   __shared__ __attribute__((aligned(16))) uint16_t act[TT * RS];
   __shared__ uint16_t slut[kLut];                  // log-cost table as bf16
   __shared__ int16_t sc[2][TT][6];                 // z1 x1 y1 z2 x2 y2, this tile's and the next's
@@ -1009,11 +1010,11 @@ __global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
     rows += nrow;
     prop_hidden<ZT, TT, NW>(A, act, a0, wv, lane);
     prop_out_topk<ZT, TT, NW>(A, act, smask, sfb, wv, lane);
-    if (wv < 4) {
+    if (wv < NC) {
       prop_cost_pair<ZT, TT>(A, nrow, 64 * (wv >> 1) + lane, wv & 1, sc[cur], smask, sfb, skey);
     } else if (tn < ntile) {
       const int64_t n0 = tn * TT;
-      prop_tile_features<ZT, TT, 2>(A, n0, (int)(A.P - n0 < TT ? A.P - n0 : TT), tl - 4 * 64, act, slut, sc[cur ^ 1]);
+      prop_tile_features<ZT, TT, 2>(A, n0, (int)(A.P - n0 < TT ? A.P - n0 : TT), tl - NC * 64, act, slut, sc[cur ^ 1]);
     }
     __syncthreads();
     if (tl < nrow && prop_finish_pair<ZT, TT>(A, t0, tl, sc[cur], sfb, skey)) atomicAdd(&nfb, 1);
@@ -1048,17 +1049,18 @@ PropFn prop_kernel_t(int Z) {
 template <bool ROUTE>
 PropFn prop_kernel(int Z) { return prop_kernel_t<ROUTE, kTile>(Z); }
 
+constexpr int kTilesTT = 64;  // pairs per tile of the persistent routing kernel (4 wavefronts)
 PropFn prop_tiles_kernel(int Z) {
   switch (Z) {
-    case 3: return k_prop_tiles<3, kTile>;
-    case 4: return k_prop_tiles<4, kTile>;
-    case 5: return k_prop_tiles<5, kTile>;
-    case 6: return k_prop_tiles<6, kTile>;
-    case 7: return k_prop_tiles<7, kTile>;
-    case 8: return k_prop_tiles<8, kTile>;
-    case 9: return k_prop_tiles<9, kTile>;
-    case 10: return k_prop_tiles<10, kTile>;
-    case 11: return k_prop_tiles<11, kTile>;
+    case 3: return k_prop_tiles<3, kTilesTT>;
+    case 4: return k_prop_tiles<4, kTilesTT>;
+    case 5: return k_prop_tiles<5, kTilesTT>;
+    case 6: return k_prop_tiles<6, kTilesTT>;
+    case 7: return k_prop_tiles<7, kTilesTT>;
+    case 8: return k_prop_tiles<8, kTilesTT>;
+    case 9: return k_prop_tiles<9, kTilesTT>;
+    case 10: return k_prop_tiles<10, kTilesTT>;
+    case 11: return k_prop_tiles<11, kTilesTT>;
     default: return nullptr;
   }
 }
@@ -1245,12 +1247,11 @@ int mlp_enqueue_costing(gr_ctx* c, Sched& s, const BatchMeta& bm) {
     A.ctr = c->d_ctr;
     A.stats = (unsigned long long*)c->d_prop_stats;
     A.fbout = c->collect ? c->d_pair_fb : nullptr;
-    const int tiles = blocks_for(bm.P, kTile);
     if (A.ends && m->tiles_grid > 0)
-      hipLaunchKernelGGL(prop_tiles_kernel(c->g.Z), dim3(std::min(tiles, m->tiles_grid)), dim3(kTile * 4), 0, c->stream,
-                         A);
+      hipLaunchKernelGGL(prop_tiles_kernel(c->g.Z), dim3(std::min(blocks_for(bm.P, kTilesTT), m->tiles_grid)),
+                         dim3(kTilesTT * 4), 0, c->stream, A);
     else
-      hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(tiles), dim3(kTile * 4), 0, c->stream, A);
+      hipLaunchKernelGGL(prop_kernel<true>(c->g.Z), dim3(blocks_for(bm.P, kTile)), dim3(kTile * 4), 0, c->stream, A);
     HIPC(c, hipGetLastError());
     return 0;
   }
@@ -1332,7 +1333,7 @@ int gr_mlp_create(gr_ctx* c, int32_t hidden, uint64_t seed, int32_t max_samples,
     m->nf3 = m->n_op / 16;  // every padded output column (the forward pass returns them all)
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, prop_tiles_kernel(c->g.Z), kTile * 4, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, prop_tiles_kernel(c->g.Z), kTilesTT * 4, 0) != hipSuccess)
       return bail(fail(c, GR_ENODEV, "device query for the routing kernel failed"));
     m->tiles_grid = ncu * per_cu;
     if ((rc = dalloc(c, &m->W1p, (size_t)m->k_in * m->H)) || (rc = dalloc(c, &m->W2p, (size_t)m->H * m->H)) ||
