@@ -2,8 +2,8 @@
 // a 3-layer MLP scores the candidate patterns of each pin pair so that only
 // its top-k are costed exactly.  Hand-written gfx950 kernels, all on bf16 MFMA
 // (v_mfma_f32_16x16x32_bf16, fp32 accumulation): the fused routing kernel
-// k_prop (feature gather, three layers in LDS, top-k, candidate costing; one
-// launch per batch), the training GEMMs (k_gemm forward and data gradients,
+// k_prop and its persistent form k_prop_tiles (feature gather, three layers
+// in LDS, top-k, candidate costing; one launch per batch), the training GEMMs (k_gemm forward and data gradients,
 // k_gemm_tn split-K weight and bias gradients), softmax cross-entropy,
 // momentum SGD on fp32 master weights, and the feature / top-k kernels of the
 // general path (hidden widths other than 256, more than 64 candidates).
@@ -965,12 +965,14 @@ __device__ __forceinline__ void prop_tile_features(const PropArgs& A, int64_t t0
 }
 
 // The routing kernel as a persistent loop over tiles t, t + grid, ... of a
-// batch with the endpoint table: while waves 0..3 cost tile t's candidates,
-// waves 4..7 decode the next tile's pairs and gather its features into the
-// activation buffer, which the costing does not touch (its keys have their
-// own LDS array, the coordinates two buffers).  Per tile the phases are the
-// layers and the top-k (all waves), then costing beside the next tile's
-// features, then the results; each tile's arithmetic is k_prop's.
+// batch with the endpoint table: while the first NC = TT/32 waves cost tile
+// t's candidates (pair half x parity), the other NC decode the next tile's
+// pairs and gather its features into the activation buffer, which the
+// costing does not touch (its keys have their own LDS array, the coordinates
+// two buffers).  Per tile the phases are the layers and the top-k (all
+// waves), then costing beside the next tile's features, then the results;
+// each tile's arithmetic is k_prop's.  TT = 64 is used: four 4-wave
+// workgroups per CU.
 template <int ZT, int TT, int NW = TT / 16>
 __global__ __launch_bounds__(64 * NW, 4) void k_prop_tiles(PropArgs A) {
   constexpr int NT = 64 * NW;
