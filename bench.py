@@ -361,6 +361,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events")
     ap.add_argument("--learned", type=int, default=1, help="also evaluate the learned proposal (0: skip)")
+    ap.add_argument("--fused", type=int, default=1, help="also time the one-launch-per-batch form (0: skip)")
     ap.add_argument("--autotune", type=int, default=1, help="pick schedule parameters by timing (0: defaults)")
     ap.add_argument("--tuning", default="",
                     help="explicit large_threads,group_max,fork,refresh_blocks,lpat_wave[,lpat_block]")
@@ -428,6 +429,43 @@ def main():
     r.sync()
     eager_ms = (time.perf_counter() - te) * 1e3 / args.steps
     print(f"# per-kernel launches: {eager_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    # the one-launch-per-batch form (tuning fork 4: costing, trees and commit
+    # in one launch per batch, the refresh a second) over the same passes, as
+    # graph replays and kernel by kernel; its demand must equal the pass above
+    fused = None
+    if args.fused:
+        tkeys = ["large_threads", "group_max", "fork", "refresh_blocks", "lpat_wave", "lpat_block"]
+        base = {k: r.tuning().as_dict()[k] for k in tkeys}
+        d_ref = r.demand()
+        r.set_tuning(**dict(base, fork=4))
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+        same = bool(np.array_equal(r.demand(), d_ref))
+        tf = time.perf_counter()
+        for _ in range(args.steps):
+            r.reset_demand()
+            r.route_schedule()
+        r.sync()
+        f_ms = (time.perf_counter() - tf) * 1e3 / args.steps
+        tf = time.perf_counter()
+        for _ in range(args.steps):
+            r.reset_demand()
+            r.route_schedule(0, nb_all - 1)
+            r.route_schedule(nb_all - 1, 1)
+        r.sync()
+        fe_ms = (time.perf_counter() - tf) * 1e3 / args.steps
+        r.sync()  # raises if a fused launch's bounded wait timed out
+        r.set_tuning(**base)
+        r.reset_demand()
+        r.route_schedule()
+        r.sync()
+        fused = {"form": "one launch per batch for costing, trees and commit (tuning fork 4), plus the refresh",
+                 "ms_per_step": round(f_ms, 4), "nets_per_s": round(n_nets / (f_ms / 1e3), 1),
+                 "per_kernel_launches_ms_per_step": round(fe_ms, 4), "graph_speedup": round(fe_ms / f_ms, 3),
+                 "same_demand_as_default": same}
+        print(f"# fused form: {f_ms:.3f} ms/step ({fe_ms:.3f} kernel by kernel), same demand: {same}",
+              file=sys.stderr, flush=True)
 
     deg, P, F = footprint(X, Y, Z, pfx, pins)
     G = int(r.demand().sum())  # committed cells over the pass = total demand bumps
@@ -528,6 +566,7 @@ def main():
         "callers": io,
         "per_kernel_launches": {"ms_per_step": round(eager_ms, 4), "nets_per_s": round(n_nets / (eager_ms / 1e3), 1),
                                 "graph_speedup": round(eager_ms / ms_step, 3)},
+        "fused_form": fused,
         "path_bytes_per_step": bytes_path,
         "path_GBps": round(bytes_path / (ms_step / 1e3) / 1e9, 2),
     }
